@@ -1,0 +1,74 @@
+"""Loader for the in-tree HIP library ``libbhk.so`` (C ABI of include/*.h).
+
+There is no CPU fallback: if the library is missing or cannot be loaded, every
+entry point raises.  Building happens in ``blackhole_amd.build`` (called by
+``__graft_entry__.build()``), never implicitly on a GPU box.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import abi
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libbhk.so")
+
+_lib = None
+
+_D = C.c_double
+_PD = C.POINTER(C.c_double)
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libbhk.so (once) and declare every exported signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeLibraryMissing(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64 (same
+    # SONAME, libamdhip64.so.7).  Loading torch first makes the dynamic loader
+    # resolve libbhk.so's dependency to that already-loaded copy instead of
+    # mapping a second runtime from /opt/rocm, which would not see the device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = C.CDLL(LIB_PATH)
+    L.bh_version.restype = C.c_int
+    L.bh_last_error.restype = C.c_char_p
+    L.bh_trace_options_default.argtypes = [C.POINTER(abi.BH_TraceOptions)]
+    L.bh_outer_horizon.argtypes = [_D]
+    L.bh_outer_horizon.restype = _D
+    L.bh_isco_prograde.argtypes = [_D]
+    L.bh_isco_prograde.restype = _D
+    L.bh_disk_lut_build.argtypes = [_D, _D, _D, C.c_int, _PD]
+    L.bh_k2_table_build.argtypes = [C.c_int, _D, _D, _PD]
+    L.bh_volume_generate_torus.argtypes = [
+        C.POINTER(abi.BH_TorusParams), C.POINTER(C.c_float), C.POINTER(abi.BH_Volume)]
+    L.bh_trace_image_plane.argtypes = [
+        C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions),
+        C.c_void_p, C.c_int, C.POINTER(abi.BH_Volume), C.POINTER(abi.BH_K2Table),
+        C.c_void_p, C.c_void_p,
+    ]
+    L.bh_last_kernel_ms.restype = C.c_float
+    for name in (
+        "bh_version", "bh_trace_options_default", "bh_disk_lut_build",
+        "bh_k2_table_build", "bh_volume_generate_torus", "bh_trace_image_plane",
+    ):
+        if getattr(L, name).restype is None or name != "bh_version":
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().bh_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed: {msg}")

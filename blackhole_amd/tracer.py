@@ -1,0 +1,176 @@
+"""Host-side interface of the image-plane tracer (Python over the C ABI).
+
+Mirrors the reference's image-plane entry (bhbCudaTraceGeodesics,
+src/blender_bridge/blender_bridge.h:348-350: spin, observer radius,
+inclination, alpha/beta window and pixel counts in; per-pixel results out)
+on top of ``bh_trace_image_plane`` (include/bhk_trace.h).  Device memory and
+streams come from PyTorch (plumbing only); every computation runs in the HIP
+library, and there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import abi
+from ._lib import check, lib
+
+K2_N = 1024
+K2_LNTH = (math.log(0.05), math.log(2000.0))
+
+
+def default_options(**overrides) -> abi.BH_TraceOptions:
+    o = abi.BH_TraceOptions()
+    check(lib().bh_trace_options_default(C.byref(o)), "bh_trace_options_default")
+    for k, v in overrides.items():
+        if not hasattr(o, k):
+            raise AttributeError(f"BH_TraceOptions has no field {k!r}")
+        setattr(o, k, v)
+    return o
+
+
+def camera(spin: float, theta_obs_deg: float, r_obs: float, width: int, height: int,
+           half_extent: float = 15.0, record_offset: int = 0) -> abi.BH_Camera:
+    return abi.BH_Camera(spin, math.radians(theta_obs_deg), r_obs, -half_extent, half_extent,
+                         -half_extent, half_extent, width, height, record_offset)
+
+
+def pack_cameras(cams: Sequence[abi.BH_Camera]) -> int:
+    """Assign consecutive record ranges; returns the total record count."""
+    off = 0
+    for c in cams:
+        c.record_offset = off
+        off += c.width * c.height
+    return off
+
+
+def isco(spin: float) -> float:
+    return lib().bh_isco_prograde(spin)
+
+
+def outer_horizon(spin: float) -> float:
+    return lib().bh_outer_horizon(spin)
+
+
+def disk_luts(cams: Sequence[abi.BH_Camera], r_out: float, n: int = 256) -> np.ndarray:
+    """Per-camera normalised NT flux tables over [r_isco(spin), r_out] (host)."""
+    out = np.zeros((len(cams), n))
+    for i, c in enumerate(cams):
+        row = np.zeros(n)
+        check(lib().bh_disk_lut_build(c.spin, isco(c.spin), r_out, n,
+                                      row.ctypes.data_as(C.POINTER(C.c_double))),
+              "bh_disk_lut_build")
+        out[i] = row
+    return out
+
+
+def k2_table(n: int = K2_N, lo: float = K2_LNTH[0], hi: float = K2_LNTH[1]) -> np.ndarray:
+    out = np.zeros(n)
+    check(lib().bh_k2_table_build(n, lo, hi, out.ctypes.data_as(C.POINTER(C.c_double))),
+          "bh_k2_table_build")
+    return out
+
+
+def torus_params(spin: float, n_r: int = 256, n_th: int = 128, n_ph: int = 64,
+                 r_hi: float = 50.0, seed: int = 0x5EED0001, **kw) -> abi.BH_TorusParams:
+    p = abi.BH_TorusParams()
+    p.spin = spin
+    p.r_in = 6.0
+    p.r_max = 12.0
+    p.r_lo = outer_horizon(spin)
+    p.r_hi = r_hi
+    p.n_r, p.n_th, p.n_ph = n_r, n_th, n_ph
+    p.sigma = 0.5
+    p.spectral_index = -11.0 / 3.0
+    p.seed = seed
+    p.n_e_max = 1.0
+    p.theta_e_max = 30.0
+    p.b_max = 30.0
+    p.gamma_ad = 4.0 / 3.0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def generate_torus(p: abi.BH_TorusParams):
+    """Host torus grid: returns (cells float32[n_ph, n_th, n_r, 4], BH_Volume sans pointer)."""
+    cells = np.zeros((p.n_ph, p.n_th, p.n_r, 4), dtype=np.float32)
+    vol = abi.BH_Volume()
+    check(lib().bh_volume_generate_torus(C.byref(p), cells.ctypes.data_as(C.POINTER(C.c_float)),
+                                         C.byref(vol)), "bh_volume_generate_torus")
+    return cells, vol
+
+
+@dataclass
+class DeviceScene:
+    """Device-resident inputs of one trace (tensors keep the memory alive)."""
+    cams: list
+    opts: abi.BH_TraceOptions
+    n_records: int
+    luts: Optional[object] = None      # torch float64 [n_cams, lut_n] on cuda
+    lut_n: int = 0
+    cells: Optional[object] = None     # torch float32 [.., 4] on cuda
+    vol: Optional[abi.BH_Volume] = None
+    k2tab: Optional[object] = None     # torch float64 [n] on cuda
+    k2: Optional[abi.BH_K2Table] = None
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device visible: the tracer has no CPU path")
+    return torch
+
+
+def upload_scene(cams, opts, luts: Optional[np.ndarray] = None, cells: Optional[np.ndarray] = None,
+                 vol: Optional[abi.BH_Volume] = None, k2tab: Optional[np.ndarray] = None,
+                 device: str = "cuda:0") -> DeviceScene:
+    torch = _torch()
+    n = max(c.record_offset + c.width * c.height for c in cams)
+    sc = DeviceScene(cams=list(cams), opts=opts, n_records=n)
+    if luts is not None:
+        sc.luts = torch.from_numpy(np.ascontiguousarray(luts, dtype=np.float64)).to(device)
+        sc.lut_n = luts.shape[-1]
+    if cells is not None:
+        sc.cells = torch.from_numpy(np.ascontiguousarray(cells)).to(device)
+        sc.vol = abi.BH_Volume()
+        C.pointer(sc.vol)[0] = vol
+        sc.vol.cells = sc.cells.data_ptr()
+    if k2tab is not None:
+        sc.k2tab = torch.from_numpy(np.ascontiguousarray(k2tab, dtype=np.float64)).to(device)
+        sc.k2 = abi.BH_K2Table(sc.k2tab.data_ptr(), k2tab.size, 0, K2_LNTH[0], K2_LNTH[1])
+    return sc
+
+
+def alloc_records(n: int, device: str = "cuda:0"):
+    torch = _torch()
+    return torch.empty((n, 128), dtype=torch.uint8, device=device)
+
+
+def launch(sc: DeviceScene, out, stream=None) -> None:
+    """Asynchronous trace of every pixel of sc.cams into out (uint8 [n, 128] cuda)."""
+    torch = _torch()
+    assert out.dtype == torch.uint8 and out.is_cuda and out.shape[1] == 128
+    assert out.shape[0] >= sc.n_records
+    cam_arr = (abi.BH_Camera * len(sc.cams))(*sc.cams)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    rc = lib().bh_trace_image_plane(
+        cam_arr, len(sc.cams), C.byref(sc.opts),
+        C.c_void_p(sc.luts.data_ptr()) if sc.luts is not None else None, sc.lut_n,
+        C.byref(sc.vol) if sc.vol is not None else None,
+        C.byref(sc.k2) if sc.k2 is not None else None,
+        C.c_void_p(out.data_ptr()), C.c_void_p(s.cuda_stream),
+    )
+    check(rc, "bh_trace_image_plane")
+
+
+def records_to_numpy(out) -> np.ndarray:
+    return out.cpu().numpy().view(abi.RECORD_DTYPE).reshape(-1)
+
+
+def last_kernel_ms() -> float:
+    return float(lib().bh_last_kernel_ms())

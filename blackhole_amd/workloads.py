@@ -1,0 +1,59 @@
+"""The five tracer workloads (cameras, options and input tables).
+
+c1  plain RK4, a = 0.9, 60 deg, r_obs = 100, 512^2 over alpha, beta in [-15, 15],
+    h = 0.01 in s, <= 2000 steps, stop at r < 1.01 r_+ or r > 200, thin NT disk
+    r_isco -> 20 M; per pixel hit class, crossing radius, g and blackbody
+    intensity (1 keV, T_max 1e7 K); FP64.
+c2  DP45 (rtol 1e-8, atol 1e-10) plus volumetric thermal synchrotron at
+    230 GHz through a Fishbone-Moncrief torus (r_in 6, r_max 12) x lognormal
+    turbulence (sigma 0.5, k^-11/3) on 256 x 128 x 64 over r in [r_+, 50];
+    a = 0.94, 17 deg, 1024^2 over alpha, beta in [-20, 20]; FP64.
+c3  precision form: a = 0.998, 85 deg, 4096^2, disk + the c2 torus recipe.
+c4  batched: 256 cameras (N ~ logU(64, 1024), a ~ U[0, 0.998],
+    theta ~ U[5, 85] deg, r_obs ~ U[50, 1000]) from a fixed seed.
+c5  surrogate evaluation frames: 2048^2 over alpha, beta in [-20, 20].
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import abi
+from .tracer import camera, default_options, disk_luts, generate_torus, k2_table, pack_cameras, torus_params
+
+KEV_HZ = 2.417989242e17
+
+
+def c1(width: int = 512, height: int = 512):
+    cams = [camera(0.9, 60.0, 100.0, width, height, 15.0)]
+    opts = default_options(
+        integrator=abi.INTEGRATOR_RK4, precision=abi.PRECISION_FP64, max_steps=2000,
+        disk_enabled=1, volume_enabled=0, h_fixed=0.01, horizon_factor=1.01, r_escape=200.0,
+        disk_r_out=20.0, nu_obs=KEV_HZ, disk_T_max=1.0e7)
+    return dict(cams=cams, opts=opts, luts=disk_luts(cams, 20.0), cells=None, vol=None, k2tab=None)
+
+
+def c2(width: int = 1024, height: int = 1024, grid=(256, 128, 64), seed: int = 0x5EED0002):
+    cams = [camera(0.94, 17.0, 100.0, width, height, 20.0)]
+    opts = default_options(
+        integrator=abi.INTEGRATOR_DP45, precision=abi.PRECISION_FP64, max_steps=20000,
+        max_attempts=100000, disk_enabled=0, volume_enabled=1, rtol=1e-8, atol=1e-10,
+        h_init=0.01, h_max=0.1, h_vol=0.01, horizon_factor=1.01, r_escape=200.0,
+        nu_obs=230.0e9)
+    tp = torus_params(0.94, n_r=grid[0], n_th=grid[1], n_ph=grid[2], seed=seed)
+    cells, vol = generate_torus(tp)
+    return dict(cams=cams, opts=opts, luts=None, cells=cells, vol=vol, k2tab=k2_table(), torus=tp)
+
+
+def c4_cameras(n: int = 256, seed: int = 0x5EED0004):
+    rng = np.random.default_rng(seed)
+    cams = []
+    for _ in range(n):
+        side = int(round(math.exp(rng.uniform(math.log(64), math.log(1024)))))
+        spin = float(rng.uniform(0.0, 0.998))
+        inc = float(rng.uniform(5.0, 85.0))
+        r_obs = float(rng.uniform(50.0, 1000.0))
+        cams.append(camera(spin, inc, r_obs, side, side, 15.0))
+    pack_cameras(cams)
+    return cams

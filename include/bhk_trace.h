@@ -1,0 +1,245 @@
+/*
+ * bhk_trace.h - C ABI of the MI355X per-pixel Kerr null-geodesic tracer.
+ *
+ * Plain C (C99), POD structs only, no torch or HIP types in any signature.
+ * Geometric units throughout: G = c = M = 1 (lengths in M); angles in rad.
+ *
+ * Each ray is an image-plane pixel of a distant-observer camera: impact
+ * parameters (alpha, beta) map to the constants of motion of a null geodesic
+ * (E = 1, xi = L/E, eta = Q/E^2; Bardeen 1973), and the geodesic is integrated
+ * backwards from the observer in Boyer-Lindquist coordinates.
+ *
+ * Reference interfaces this layer replaces or completes:
+ *   - src/cuda/kernel_launch.h:145-148  bh_launch_geodesic_kernel (render ABI;
+ *     mirrored in bh_kernel_launch.h)
+ *   - src/blender_bridge/blender_bridge.h:165-168,348-350
+ *     bhbTraceGeodesicsImagePlane / bhbCudaTraceGeodesics (image-plane tracing;
+ *     the reference CUDA entry returns -1, blender_bridge.cpp:709-714)
+ *   - src/physics/raytracer.h:46-79  RayStatus / RayTraceResult (hit classes,
+ *     per-ray result record)
+ *   - src/physics/verified/rk4.hpp:247-260 rk4_step and
+ *     src/physics/verified/rk4.h:182-299 dp45Step / dp45NextStep (integrators)
+ *
+ * Error convention (as the reference's C entry points, kernel_launch.cu:220-269):
+ * every function returns 0 on success and -1 on failure, never throws across
+ * the ABI, and leaves a message retrievable with bh_last_error().
+ * Launches are asynchronous on the given stream (NULL = default stream); the
+ * caller synchronises.  All launch parameters travel by value in kernel
+ * arguments, so concurrent calls on different streams are safe.
+ */
+#ifndef BHK_TRACE_H
+#define BHK_TRACE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BHK_VERSION_MAJOR 0
+#define BHK_VERSION_MINOR 1
+
+/* ---------------------------------------------------------------------- */
+/* Enumerations                                                            */
+/* ---------------------------------------------------------------------- */
+
+/* Hit classes; the first four follow RayStatus (src/physics/raytracer.h:46-52):
+ * Captured -> HORIZON, Escaped -> ESCAPED, MaxSteps -> MAX_STEPS, plus the
+ * disk hit of HitResult (src/cuda/device_physics.cuh:806-818). */
+enum BH_HitClass {
+    BH_HIT_NONE      = 0, /* ray not traced (e.g. skipped by a mask)            */
+    BH_HIT_HORIZON   = 1, /* r < horizon_factor * r_+                           */
+    BH_HIT_DISK      = 2, /* crossed the thin disk inside [r_isco, disk_r_out]  */
+    BH_HIT_ESCAPED   = 3, /* r > r_escape                                       */
+    BH_HIT_MAX_STEPS = 4, /* step (RK4) or attempt (DP45) budget exhausted      */
+    BH_HIT_OPAQUE    = 5  /* volume transmittance fell below transmit_min       */
+};
+
+enum BH_Integrator {
+    BH_INTEGRATOR_RK4  = 0, /* classic fixed-step RK4 in s                       */
+    BH_INTEGRATOR_DP45 = 1  /* Dormand-Prince 5(4), FSAL, per-component rtol/atol */
+};
+
+enum BH_Precision {
+    BH_PRECISION_FP64 = 0,  /* FP64 throughout                                   */
+    BH_PRECISION_FP32 = 1   /* FP32 stepping, FP64 islands (setup, turning
+                               points, r < island_r_factor * r_+)              */
+};
+
+/* Per-record flag bits (BH_PixelRecord.flags). */
+#define BH_FLAG_FP64_ISLAND   0x01u /* at least one step ran in an FP64 island   */
+#define BH_FLAG_FALLBACK      0x02u /* surrogate deferred this ray to the tracer */
+#define BH_FLAG_AUDITED       0x04u /* ray traced for the surrogate audit        */
+#define BH_FLAG_SURROGATE     0x08u /* record produced by the surrogate          */
+
+/* ---------------------------------------------------------------------- */
+/* Inputs                                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* One distant-observer camera.  Pixel (px, py) has centre
+ *   alpha = alpha_min + (px + 0.5) * (alpha_max - alpha_min) / width
+ *   beta  = beta_max  - (py + 0.5) * (beta_max  - beta_min)  / height
+ * (row 0 at the top).  Records of this camera occupy
+ * [record_offset, record_offset + width*height) of the output array,
+ * row-major. */
+typedef struct BH_Camera {
+    double  spin;          /* a/M in [0, 1)                                    */
+    double  theta_obs;     /* inclination from the +z spin axis [rad], (0, pi) */
+    double  r_obs;         /* observer Boyer-Lindquist radius [M]              */
+    double  alpha_min, alpha_max;
+    double  beta_min, beta_max;
+    int32_t width, height;
+    int64_t record_offset;
+} BH_Camera; /* 72 bytes */
+
+/* Integration and radiative-transfer options (bh_trace_options_default). */
+typedef struct BH_TraceOptions {
+    int32_t integrator;     /* BH_Integrator                                    */
+    int32_t precision;      /* BH_Precision                                     */
+    int32_t max_steps;      /* accepted-step budget                             */
+    int32_t max_attempts;   /* DP45 attempt budget (accepted + rejected)        */
+    int32_t disk_enabled;   /* thin Novikov-Thorne disk in the equatorial plane */
+    int32_t volume_enabled; /* volumetric synchrotron transfer through BH_Volume*/
+    int32_t max_crossings;  /* equatorial crossings recorded (<= 3)            */
+    int32_t refill_batch;   /* steps between lane refills in the persistent
+                               kernel (0 = library default)                     */
+    double  h_fixed;        /* RK4 step in s, where d(lambda_Mino) = ds / r     */
+    double  h_init;         /* DP45 first trial step                            */
+    double  h_max;          /* DP45 step cap outside the volume support         */
+    double  h_vol;          /* DP45 step cap inside the volume support          */
+    double  rtol, atol;     /* DP45 tolerances                                  */
+    double  horizon_factor; /* stop when r < horizon_factor * r_+               */
+    double  r_escape;       /* stop when r > r_escape                           */
+    double  disk_r_out;     /* thin-disk outer edge [M]; inner edge is r_isco   */
+    double  nu_obs;         /* observing frequency at infinity [Hz]             */
+    double  disk_T_max;     /* disk effective temperature at the flux peak [K]  */
+    double  length_unit_cm; /* GM/c^2 in cm (converts path lengths)             */
+    double  transmit_min;   /* stop when volume transmittance < this (0 = off)  */
+    double  island_r_factor;/* FP32 form: FP64 steps when r < this * r_+        */
+    double  island_turn_dist;/* FP32 form: FP64 steps within this [M] of a
+                               radial or polar turning point                    */
+} BH_TraceOptions;
+
+/* Synthetic GRMHD-like volume on a Boyer-Lindquist (r, theta, phi) grid.
+ * cells: device pointer to n_r*n_th*n_ph float4 {n_e [cm^-3], Theta_e
+ * (k T_e / m_e c^2), |B| [G], 0}; index ((iph * n_th) + ith) * n_r + ir.
+ * Cell-centred samples: cell i of an axis of n cells spans
+ * [lo + i*d, lo + (i+1)*d]; r clamps, theta clamps (mirror at the poles is
+ * the same for the half-cell overhang), phi wraps (2*pi periodic) -- the
+ * address modes of src/cuda/device_physics.cuh:1775-1829. */
+typedef struct BH_Volume {
+    const void* cells;      /* device float4[n_r*n_th*n_ph]                     */
+    int32_t n_r, n_th, n_ph, pad0;
+    double  r_min, r_max;   /* radial extent of the grid [M]                    */
+    double  sup_r_lo, sup_r_hi, sup_u_max; /* support of n_e > 0:
+                               r in [lo, hi], |cos theta| <= u_max              */
+    double  ell;            /* constant specific angular momentum -u_phi/u_t of
+                               the fluid (rotation law of the torus)            */
+} BH_Volume;
+
+/* Thermal-synchrotron K_2 table: tab[i] = ln(K_2(1/Theta) / (2 Theta^2)) at
+ * ln Theta = lnth_min + i * (lnth_max - lnth_min) / (n - 1); linear
+ * interpolation, clamped outside. */
+typedef struct BH_K2Table {
+    const double* tab;      /* device double[n]                                 */
+    int32_t n, pad0;
+    double  lnth_min, lnth_max;
+} BH_K2Table;
+
+/* ---------------------------------------------------------------------- */
+/* Output                                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* One record per pixel (128 bytes).  Crossing slots beyond n_cross are 0. */
+typedef struct BH_PixelRecord {
+    double   intensity;     /* I_nu at the observer [erg s^-1 cm^-2 Hz^-1 sr^-1] */
+    double   r_exit;        /* r at termination [M]                             */
+    double   u_exit;        /* cos(theta) at termination                        */
+    double   phi_exit;      /* Boyer-Lindquist phi at termination [rad]         */
+    double   cross_r[3];    /* radii of the first equatorial crossings [M]      */
+    double   cross_g[3];    /* disk redshift g = nu_obs/nu_emit at each crossing
+                               (Keplerian prograde orbit; 0 inside r_isco)     */
+    double   tau;           /* optical depth accumulated through the volume     */
+    double   drift_null;    /* |v_r^2 - R(r)| / max(R_scale) at termination     */
+    double   drift_carter;  /* |v_u^2 - U(u)| / max(U_scale) at termination     */
+    int32_t  steps;         /* accepted steps                                   */
+    int32_t  attempts;      /* attempts (accepted + rejected; RK4: = steps)     */
+    uint8_t  hit_class;     /* BH_HitClass                                      */
+    uint8_t  n_cross;       /* number of equatorial crossings recorded          */
+    uint8_t  flags;         /* BH_FLAG_*                                        */
+    uint8_t  pad0;
+    float    confidence;    /* surrogate confidence (1 for traced rays)         */
+    int32_t  fp64_steps;    /* FP32 form: steps taken in FP64 islands           */
+    int32_t  n_samples;     /* volume samples taken along the ray             */
+} BH_PixelRecord;
+
+/* ---------------------------------------------------------------------- */
+/* Entry points                                                            */
+/* ---------------------------------------------------------------------- */
+
+/* Library version; returns BHK_VERSION_MAJOR * 100 + BHK_VERSION_MINOR. */
+int bh_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* bh_last_error(void);
+
+/* Fill *opts with the defaults (RK4, FP64, h_fixed 0.01, 2000 steps,
+ * horizon_factor 1.01, r_escape 200, disk to 20 M, nu 230 GHz, ...). */
+int bh_trace_options_default(BH_TraceOptions* opts);
+
+/* Outer horizon r_+ = 1 + sqrt(1 - a^2) and prograde ISCO (Bardeen, Press &
+ * Teukolsky 1972), as src/physics/kerr.h:139-229 with M = 1. */
+double bh_outer_horizon(double spin);
+double bh_isco_prograde(double spin);
+
+/* Host: the normalised Novikov-Thorne flux table of the thin disk,
+ * F(r) = (1 - sqrt(r_in/r)) (1 + a/(2 sqrt r)) / r^3 over n equally spaced
+ * radii in [r_in, r_out], divided by its largest sample
+ * (src/physics/lut.h:50-86, src/physics/thin_disk.h:243-266 with M = 1). */
+int bh_disk_lut_build(double spin, double r_in, double r_out, int n, double* out);
+
+/* Host: the K_2 table of BH_K2Table (n entries over [lnth_min, lnth_max]). */
+int bh_k2_table_build(int n, double lnth_min, double lnth_max, double* out);
+
+/* Host: synthetic torus volume (Fishbone & Moncrief 1976 constant-l torus,
+ * modulated by a seeded lognormal Gaussian random field with a k^(-11/3)
+ * spectrum).  cells_host: float[4 * n_r*n_th*n_ph]; fills vol's scalar fields
+ * (not vol->cells). */
+typedef struct BH_TorusParams {
+    double  spin;
+    double  r_in, r_max;    /* torus inner edge and pressure maximum [M]        */
+    double  r_lo, r_hi;     /* grid radial extent [M]                            */
+    int32_t n_r, n_th, n_ph, pad0;
+    double  sigma;          /* lognormal width of the turbulence                 */
+    double  spectral_index; /* P(k) ~ k^spectral_index (-11/3)                  */
+    uint64_t seed;
+    double  n_e_max;        /* peak electron density [cm^-3]                     */
+    double  theta_e_max;    /* peak dimensionless electron temperature           */
+    double  b_max;          /* field strength at the pressure maximum [G]        */
+    double  gamma_ad;       /* adiabatic index (4/3)                             */
+} BH_TorusParams;
+int bh_volume_generate_torus(const BH_TorusParams* p, float* cells_host, BH_Volume* vol);
+
+/* Trace every pixel of n_cameras cameras into out (device array of
+ * BH_PixelRecord, caller-owned, sized by the cameras' record ranges).
+ * cams and opts are host pointers.  disk_luts: device double array holding
+ * n_cameras tables of disk_lut_n entries (camera c uses
+ * disk_luts + c*disk_lut_n, spanning [r_isco(spin_c), opts->disk_r_out]);
+ * may be NULL when the disk is disabled.  vol / k2 may be NULL when the
+ * volume is disabled.  Asynchronous on stream. */
+int bh_trace_image_plane(const BH_Camera* cams, int n_cameras,
+                         const BH_TraceOptions* opts,
+                         const double* disk_luts, int disk_lut_n,
+                         const BH_Volume* vol, const BH_K2Table* k2,
+                         BH_PixelRecord* out, void* stream);
+
+/* Elapsed time of the last bh_trace_image_plane kernel on this thread, from
+ * HIP events recorded around the launch on its stream (ms; -1 if none).  The
+ * caller must have synchronised the stream. */
+float bh_last_kernel_ms(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BHK_TRACE_H */

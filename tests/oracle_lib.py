@@ -1,0 +1,122 @@
+"""ctypes loader for the FP64 CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use this module, as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from blackhole_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_D = C.c_double
+_PD = C.POINTER(C.c_double)
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_trace.argtypes = [
+            C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions),
+            _PD, C.c_int, C.POINTER(abi.BH_Volume), C.POINTER(C.c_float),
+            C.POINTER(abi.BH_K2Table), _PD, C.POINTER(C.c_int64), C.c_int64,
+            C.c_void_p, C.c_int,
+        ]
+        L.orc_trace.restype = C.c_int
+        L.orc_rhs.argtypes = [_D, _D, _D, _PD, _PD]
+        L.orc_rk4_step.argtypes = [_D, _D, _D, _PD, _D, _PD]
+        L.orc_dp45_attempt.argtypes = [_D, _D, _D, _PD, _D, _D, _D, _PD]
+        L.orc_dp45_attempt.restype = _D
+        L.orc_pow_m0p2.argtypes = [_D]
+        L.orc_pow_m0p2.restype = _D
+        L.orc_dp45_factor.argtypes = [_D, C.c_int, C.c_int]
+        L.orc_dp45_factor.restype = _D
+        L.orc_setup.argtypes = [C.POINTER(abi.BH_Camera), C.POINTER(abi.BH_TraceOptions),
+                                C.c_int64, C.c_int64, _PD, _PD]
+        L.orc_potentials.argtypes = [_D, _D, _D, _D, _D, _PD]
+        for name in ("orc_outer_horizon", "orc_isco"):
+            getattr(L, name).argtypes = [_D]
+            getattr(L, name).restype = _D
+        L.orc_photon_orbit.argtypes = [_D, C.c_int]
+        L.orc_photon_orbit.restype = _D
+        L.orc_critical_params.argtypes = [_D, _D, _PD]
+        L.orc_disk_g.argtypes = [_D, _D, _D]
+        L.orc_disk_g.restype = _D
+        L.orc_fluid_g.argtypes = [_D, _D, _D, _D, _D]
+        L.orc_fluid_g.restype = _D
+        L.orc_planck.argtypes = [_D, _D]
+        L.orc_planck.restype = _D
+        L.orc_synch_j.argtypes = [_D, _D, _D, _D, _D]
+        L.orc_synch_j.restype = _D
+        L.orc_disk_intensity.argtypes = [_D, _D, _D, _D]
+        L.orc_disk_intensity.restype = _D
+        L.orc_crossing_radius.argtypes = [_PD, _PD, _PD, _PD, _D]
+        L.orc_crossing_radius.restype = _D
+        L.orc_lut_linear.argtypes = [_PD, C.c_int, _D, _D, _D]
+        L.orc_lut_linear.restype = _D
+        L.orc_volume_fetch.argtypes = [C.POINTER(abi.BH_Volume), C.POINTER(C.c_float), _D, _D, _D, _PD]
+        L.orc_disk_lut.argtypes = [_D, _D, _D, C.c_int, _PD]
+        L.orc_bessel_k.argtypes = [_D, _D]
+        L.orc_bessel_k.restype = _D
+        L.orc_k2_table.argtypes = [C.c_int, _D, _D, _PD]
+        L.orc_rte_step.argtypes = [_PD, _D, _D, _D]
+        _lib = L
+    return _lib
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_PD)
+
+
+def disk_lut(spin: float, r_in: float, r_out: float, n: int = 256) -> np.ndarray:
+    out = np.zeros(n)
+    assert lib().orc_disk_lut(spin, r_in, r_out, n, dptr(out)) == 0
+    return out
+
+
+def k2_table(n: int, lo: float, hi: float) -> np.ndarray:
+    out = np.zeros(n)
+    assert lib().orc_k2_table(n, lo, hi, dptr(out)) == 0
+    return out
+
+
+def trace(cams, opts, luts=None, lut_n=0, vol=None, cells=None, k2=None, k2_tab=None,
+          ids=None, nthreads=0) -> np.ndarray:
+    """Trace with the oracle; returns a RECORD_DTYPE array (one per id, or all rays)."""
+    cam_arr = (abi.BH_Camera * len(cams))(*cams)
+    total = max(c.record_offset + c.width * c.height for c in cams)
+    n = total if ids is None else len(ids)
+    out = np.zeros(n, dtype=abi.RECORD_DTYPE)
+    ids_arr = None
+    if ids is not None:
+        ids_arr = np.ascontiguousarray(ids, dtype=np.int64)
+    rc = lib().orc_trace(
+        cam_arr, len(cams), C.byref(opts),
+        dptr(luts) if luts is not None else None, lut_n,
+        C.byref(vol) if vol is not None else None,
+        cells.ctypes.data_as(C.POINTER(C.c_float)) if cells is not None else None,
+        C.byref(k2) if k2 is not None else None,
+        dptr(k2_tab) if k2_tab is not None else None,
+        ids_arr.ctypes.data_as(C.POINTER(C.c_int64)) if ids_arr is not None else None,
+        n if ids is not None else 0,
+        out.ctypes.data_as(C.c_void_p), nthreads,
+    )
+    assert rc == 0
+    return out

@@ -1,0 +1,59 @@
+"""Shared helpers for GPU-vs-oracle parity tests."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from blackhole_amd import abi, tracer
+
+
+def k2_struct(k2tab: np.ndarray) -> abi.BH_K2Table:
+    return abi.BH_K2Table(None, k2tab.size, 0, tracer.K2_LNTH[0], tracer.K2_LNTH[1])
+
+
+def run_gpu(w: dict) -> np.ndarray:
+    """Trace workload dict w on cuda:0 through the C ABI; returns records (numpy)."""
+    import torch
+    sc = tracer.upload_scene(w["cams"], w["opts"], luts=w.get("luts"), cells=w.get("cells"),
+                             vol=w.get("vol"), k2tab=w.get("k2tab"))
+    out = tracer.alloc_records(sc.n_records)
+    out.fill_(0)
+    tracer.launch(sc, out)
+    torch.cuda.synchronize()
+    return tracer.records_to_numpy(out)
+
+
+def run_oracle(oracle, w: dict, ids=None) -> np.ndarray:
+    cells = w.get("cells")
+    return oracle.trace(
+        w["cams"], w["opts"], luts=w.get("luts"), lut_n=(w["luts"].shape[-1] if w.get("luts") is not None else 0),
+        vol=w.get("vol"), cells=cells.reshape(-1) if cells is not None else None,
+        k2=k2_struct(w["k2tab"]) if w.get("k2tab") is not None else None, k2_tab=w.get("k2tab"),
+        ids=ids)
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    d = np.abs(a - b)
+    s = np.maximum(np.abs(b), 1e-300)
+    return np.where(d == 0, 0.0, d / s)
+
+
+def compare(gpu: np.ndarray, ref: np.ndarray, tol: float = 1e-9) -> dict:
+    """Field-by-field comparison; returns summary stats (also used by bench)."""
+    same_class = gpu["hit_class"] == ref["hit_class"]
+    out = {
+        "n": int(ref.size),
+        "class_mismatch": int((~same_class).sum()),
+        "bitwise_exit": float(np.mean((gpu["r_exit"] == ref["r_exit"]) & (gpu["u_exit"] == ref["u_exit"])
+                                      & (gpu["phi_exit"] == ref["phi_exit"]))),
+    }
+    m = same_class
+    for f in ("r_exit", "u_exit", "phi_exit", "intensity", "tau"):
+        r = rel(gpu[f][m], ref[f][m])
+        out[f"max_rel_{f}"] = float(r.max()) if r.size else 0.0
+    out["steps_mismatch"] = int((gpu["steps"][m] != ref["steps"][m]).sum())
+    out["attempts_mismatch"] = int((gpu["attempts"][m] != ref["attempts"][m]).sum())
+    return out

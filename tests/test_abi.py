@@ -1,0 +1,99 @@
+"""C-ABI checks that need no GPU: the library loads, exports every declared
+symbol, and its host-side entry points and argument validation behave."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from blackhole_amd import abi, tracer
+from blackhole_amd._lib import LIB_PATH, lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(bhb?_?\w+|bh\w+)\s*\(", txt, flags=re.M):
+            name = m.group(1)
+            if name.startswith(("bh", "bhb")) and not name.isupper():
+                names.append(name)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = lib()
+    names = declared_functions()
+    assert "bh_trace_image_plane" in names
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_version_and_defaults():
+    assert lib().bh_version() == 1
+    o = tracer.default_options()
+    assert o.integrator == abi.INTEGRATOR_RK4 and o.max_steps == 2000
+    assert o.horizon_factor == 1.01 and o.r_escape == 200.0 and o.disk_r_out == 20.0
+
+
+def test_closed_forms_match_oracle(oracle):
+    for a in (0.0, 0.5, 0.9, 0.94, 0.998):
+        assert tracer.isco(a) == oracle.lib().orc_isco(a)
+        assert tracer.outer_horizon(a) == oracle.lib().orc_outer_horizon(a)
+
+
+def test_disk_lut_matches_oracle_bitwise(oracle):
+    for a in (0.0, 0.9, 0.998):
+        ri = tracer.isco(a)
+        cam = tracer.camera(a, 60.0, 100.0, 4, 4)
+        got = tracer.disk_luts([cam], 20.0)[0]
+        ref = oracle.disk_lut(a, ri, 20.0, 256)
+        assert got.tobytes() == ref.tobytes()
+
+
+def test_k2_table_matches_oracle(oracle):
+    got = tracer.k2_table(64)
+    ref = oracle.k2_table(64, tracer.K2_LNTH[0], tracer.K2_LNTH[1])
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("field,value", [
+    ("integrator", 7), ("max_crossings", 4), ("h_fixed", 0.0), ("precision", 9)])
+def test_invalid_options_rejected_without_gpu(field, value):
+    o = tracer.default_options()
+    setattr(o, field, value)
+    cams = (abi.BH_Camera * 1)(tracer.camera(0.9, 60.0, 100.0, 8, 8))
+    rc = lib().bh_trace_image_plane(cams, 1, C.byref(o), C.c_void_p(1), 256, None, None,
+                                    C.c_void_p(1), None)
+    assert rc == -1
+    assert lib().bh_last_error()
+
+
+@pytest.mark.parametrize("spin,theta,w", [(1.2, 1.0, 8), (0.5, 0.0, 8), (0.5, 1.0, 0)])
+def test_invalid_camera_rejected_without_gpu(spin, theta, w):
+    o = tracer.default_options()
+    cams = (abi.BH_Camera * 1)(abi.BH_Camera(spin, theta, 100.0, -1, 1, -1, 1, w, 8, 0))
+    rc = lib().bh_trace_image_plane(cams, 1, C.byref(o), C.c_void_p(1), 256, None, None,
+                                    C.c_void_p(1), None)
+    assert rc == -1
+
+
+def test_null_arguments_rejected():
+    o = tracer.default_options()
+    assert lib().bh_trace_image_plane(None, 0, C.byref(o), None, 0, None, None, None, None) == -1
+    cams = (abi.BH_Camera * 1)(tracer.camera(0.9, 60.0, 100.0, 8, 8))
+    assert lib().bh_trace_image_plane(cams, 1, None, None, 0, None, None, C.c_void_p(1), None) == -1
+    # disk enabled but no LUT
+    assert lib().bh_trace_image_plane(cams, 1, C.byref(o), None, 0, None, None, C.c_void_p(1), None) == -1
+    # volume enabled but no volume
+    o.volume_enabled = 1
+    assert lib().bh_trace_image_plane(cams, 1, C.byref(o), C.c_void_p(1), 256, None, None,
+                                      C.c_void_p(1), None) == -1
