@@ -55,7 +55,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=8)
+    ap.add_argument("--cpu-stride", type=int, default=1)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 512 MiB device copy (profiler counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -90,6 +92,13 @@ def main() -> int:
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     rays = sc.n_records
     value = rays / (ms_per_step / 1000.0)
+
+    if args.copy_calibration:
+        src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     rec = tracer.records_to_numpy(out)
     f = flops.frame_flops(rec, w["opts"].integrator)

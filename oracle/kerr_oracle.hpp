@@ -326,7 +326,7 @@ inline T dp45_attempt(const RayConst<T>& c, const T* y, const T* k1, T h, T rtol
 
 // x^(-1/5) for x > 0 from IEEE operations only (exact frexp/ldexp split,
 // chord start on [0.5, 1), two Newton steps y <- y (6 - m y^5) / 5); relative
-// error < 1e-7, ample for a step-size controller, and bit-reproducible.
+// error < 2e-6, ample for a step-size controller, and bit-reproducible.
 template <class T>
 inline T pow_m0p2(T x) {
   int e = 0;

@@ -1,0 +1,90 @@
+// ref_wrap.cpp - extern "C" wrapper exposing a few of the reference's own
+// FP64 physics functions (compiled from /root/reference/src/physics) so the
+// oracle can be checked against them.  Test infrastructure only; built by
+// oracle/build_ref.sh into oracle/_ref/libref.so.
+#include <cmath>
+
+#include "physics/analytic_kerr_geodesic.h"
+#include "physics/iron_kline.h"
+#include "physics/kerr.h"
+#include "physics/rte_integrator.h"
+#include "physics/verified/geodesic.hpp"
+
+namespace {
+// kerr.h works in CGS with mass in grams; M = 1 (geometric) is mass = c^2/G.
+constexpr double kMassM1 = physics::C2 / physics::G;
+}  // namespace
+
+extern "C" {
+
+void ref_kerr_potentials(double r, double theta, double a, double e, double lz, double q,
+                         double* out4) {
+  physics::KerrGeodesicConsts c{e, lz, q};
+  const physics::KerrPotentials p = physics::kerrPotentials(r, theta, kMassM1, a, c);
+  out4[0] = p.rPot;
+  out4[1] = p.dRdr;
+  out4[2] = p.thetaPot;
+  out4[3] = p.dThetadtheta;
+}
+
+double ref_carter_constant(double theta, double a, double e, double lz, double ptheta) {
+  return verified::carter_constant(theta, a, e, lz, ptheta);
+}
+
+void ref_critical_params(double r, double a, double* xi_eta) {
+  const physics::ImpactParams ip = physics::criticalImpactParams(r, a);
+  xi_eta[0] = ip.xi;
+  xi_eta[1] = ip.eta;
+}
+
+double ref_photon_orbit(double a, int prograde) {
+  return prograde ? physics::progradePhotonOrbit(a) : physics::retrogradePhotonOrbit(a);
+}
+
+double ref_isco_geom(double a) { return physics::iscoGeom(a); }
+
+double ref_kerr_isco(double a, int prograde) {
+  return physics::kerrIscoRadius(kMassM1, a * physics::G * kMassM1 / physics::C2, prograde != 0) /
+         (physics::G * kMassM1 / physics::C2);
+}
+
+double ref_outer_horizon(double a) {
+  const double m = physics::G * kMassM1 / physics::C2;
+  return physics::kerrOuterHorizon(kMassM1, a * m) / m;
+}
+
+double ref_disk_g(double r, double phi, double a, double iota) {
+  return physics::kerrDiskGFactor(r, phi, a, iota);
+}
+
+void ref_rte_step(double* state3, double j, double alpha, double ds) {
+  physics::RteState s{state3[0], state3[1], state3[2]};
+  s = physics::rteStep(s, j, alpha, ds);
+  state3[0] = s.iNu;
+  state3[1] = s.tau;
+  state3[2] = s.sCm;
+}
+
+double ref_planck(double nu, double T) { return physics::planckFunction(nu, T); }
+
+double ref_synch_j(double nu, double B, double ne, double th) {
+  return physics::synchrotronThermalEmissivity(nu, B, ne, th);
+}
+
+double ref_synch_alpha(double nu, double B, double ne, double th) {
+  return physics::synchrotronThermalAbsorption(nu, B, ne, th);
+}
+
+void ref_kerr_step_mino(double a, double e, double lz, double q, double* state6, double dlam) {
+  physics::KerrGeodesicConsts c{e, lz, q};
+  physics::KerrGeodesicState s{state6[0], state6[1], state6[2], state6[3], state6[4], state6[5]};
+  s = physics::kerrStepMino(s, kMassM1, a, c, dlam);
+  state6[0] = s.r;
+  state6[1] = s.theta;
+  state6[2] = s.phi;
+  state6[3] = s.t;
+  state6[4] = s.signR;
+  state6[5] = s.signTheta;
+}
+
+}  // extern "C"
