@@ -73,8 +73,7 @@ def main() -> int:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
     w = workloads.c2(args.size, args.size)
-    sc = tracer.upload_scene(w["cams"], w["opts"], luts=w["luts"], cells=w["cells"], vol=w["vol"],
-                             k2tab=w["k2tab"])
+    sc = tracer.upload_scene(w["cams"], w["opts"], luts=w["luts"], cells=w["cells"], vol=w["vol"])
     out = tracer.alloc_records(sc.n_records)
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):

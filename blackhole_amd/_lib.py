@@ -49,18 +49,16 @@ def lib():
     L.bh_isco_prograde.argtypes = [_D]
     L.bh_isco_prograde.restype = _D
     L.bh_disk_lut_build.argtypes = [_D, _D, _D, C.c_int, _PD]
-    L.bh_k2_table_build.argtypes = [C.c_int, _D, _D, _PD]
     L.bh_volume_generate_torus.argtypes = [
         C.POINTER(abi.BH_TorusParams), C.POINTER(C.c_float), C.POINTER(abi.BH_Volume)]
     L.bh_trace_image_plane.argtypes = [
         C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions),
-        C.c_void_p, C.c_int, C.POINTER(abi.BH_Volume), C.POINTER(abi.BH_K2Table),
-        C.c_void_p, C.c_void_p,
+        C.c_void_p, C.c_int, C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p,
     ]
     L.bh_last_kernel_ms.restype = C.c_float
     for name in (
         "bh_version", "bh_trace_options_default", "bh_disk_lut_build",
-        "bh_k2_table_build", "bh_volume_generate_torus", "bh_trace_image_plane",
+        "bh_volume_generate_torus", "bh_trace_image_plane",
     ):
         if getattr(L, name).restype is None or name != "bh_version":
             getattr(L, name).restype = C.c_int

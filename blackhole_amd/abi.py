@@ -80,16 +80,6 @@ class BH_Volume(C.Structure):
     ]
 
 
-class BH_K2Table(C.Structure):
-    _fields_ = [
-        ("tab", C.c_void_p),
-        ("n", C.c_int32),
-        ("pad0", C.c_int32),
-        ("lnth_min", C.c_double),
-        ("lnth_max", C.c_double),
-    ]
-
-
 class BH_TorusParams(C.Structure):
     _fields_ = [
         ("spin", C.c_double),

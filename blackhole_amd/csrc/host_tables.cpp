@@ -113,16 +113,6 @@ int bh_disk_lut_build(double a, double r_in, double r_out, int n, double* out) {
   return 0;
 }
 
-int bh_k2_table_build(int n, double lo, double hi, double* out) {
-  if (!out || n < 2 || !(hi > lo)) return bhk::fail("bh_k2_table_build: invalid arguments");
-  for (int i = 0; i < n; ++i) {
-    const double lt = lo + (hi - lo) * static_cast<double>(i) / static_cast<double>(n - 1);
-    const double th = std::exp(lt);
-    out[i] = std::log(bessel_k(2.0, 1.0 / th) / (2.0 * th * th));
-  }
-  return 0;
-}
-
 int bh_volume_generate_torus(const BH_TorusParams* p, float* cells, BH_Volume* vol) {
   if (!p || !cells || !vol) return bhk::fail("bh_volume_generate_torus: NULL argument");
   const int nr = p->n_r, nt = p->n_th, np = p->n_ph;
@@ -193,6 +183,17 @@ int bh_volume_generate_torus(const BH_TorusParams* p, float* cells, BH_Volume* v
   }
   if (!(hm1_max > 0.0)) return bhk::fail("bh_volume_generate_torus: empty torus");
   const double b_max = p->b_max;
+  // K_2(1/Theta_e) / (2 Theta_e^2) per (r, theta) cell (Theta_e does not
+  // depend on phi): the thermal-synchrotron normalisation, stored in the
+  // fourth lane of each cell.
+  std::vector<double> k2r(static_cast<size_t>(nr) * nt, 0.0);
+  for (size_t i = 0; i < k2r.size(); ++i) {
+    const double x = hm1[i] / hm1_max;
+    if (x > 0.0) {
+      const double th_e = p->theta_e_max * x;
+      k2r[i] = bessel_k(2.0, 1.0 / th_e) / (2.0 * th_e * th_e);
+    }
+  }
   const double s = p->sigma;
   double sup_lo = 1e300, sup_hi = -1e300, sup_u = 0.0;
   for (int ip = 0; ip < np; ++ip) {
@@ -212,7 +213,7 @@ int bh_volume_generate_torus(const BH_TorusParams* p, float* cells, BH_Volume* v
         c[0] = static_cast<float>(p->n_e_max * rho * L);
         c[1] = static_cast<float>(p->theta_e_max * x);
         c[2] = static_cast<float>(b_max * std::sqrt(pr * L));
-        c[3] = 0.0f;
+        c[3] = static_cast<float>(k2r[static_cast<size_t>(it) * nr + ir]);
         const double r_lo = p->r_lo + ir * dr, r_hi = r_lo + dr;
         sup_lo = std::min(sup_lo, r_lo);
         sup_hi = std::max(sup_hi, r_hi);

@@ -323,32 +323,40 @@ __device__ __forceinline__ double disk_intensity(double F, double g, double nu, 
 
 // ---- volume ---------------------------------------------------------------
 
-__device__ __forceinline__ void volume_fetch(const float4* __restrict__ cells, const BH_Volume& v,
-                                             double r, double u, double phi, double* out3) {
+// Cell-centred trilinear sample of the cell fields {n_e, Theta_e, |B|,
+// K_2(1/Theta_e)/(2 Theta_e^2)} (one float4 per corner): r and theta clamp,
+// phi wraps (d_sample_grmhd, src/cuda/device_physics.cuh:1775-1829).
+__device__ __forceinline__ void volume_fetch(const float4* __restrict__ cells, int n_r, int n_th,
+                                             int n_ph, double r_min, double r_span, double r,
+                                             double u, double phi, double* out4) {
   const double theta = acos(u);
-  const double xr = (r - v.r_min) / (v.r_max - v.r_min) * static_cast<double>(v.n_r) - 0.5;
-  const double xt = theta / kPi * static_cast<double>(v.n_th) - 0.5;
-  const double xp = phi / kTwoPi * static_cast<double>(v.n_ph) - 0.5;
+  const double xr = (r - r_min) / r_span * static_cast<double>(n_r) - 0.5;
+  const double xt = theta / kPi * static_cast<double>(n_th) - 0.5;
+  const double xp = phi / kTwoPi * static_cast<double>(n_ph) - 0.5;
   const double fr0 = floor(xr), ft0 = floor(xt), fp0 = floor(xp);
   const double fr = xr - fr0, ft = xt - ft0, fp = xp - fp0;
-  long long ir0 = static_cast<long long>(fr0);
-  long long it0 = static_cast<long long>(ft0);
-  long long ip0 = static_cast<long long>(fp0);
-  long long ir1 = ir0 + 1, it1 = it0 + 1;
-  const long long nr = v.n_r, nt = v.n_th, np = v.n_ph;
-  ir0 = ir0 < 0 ? 0 : (ir0 > nr - 1 ? nr - 1 : ir0);
-  ir1 = ir1 < 0 ? 0 : (ir1 > nr - 1 ? nr - 1 : ir1);
-  it0 = it0 < 0 ? 0 : (it0 > nt - 1 ? nt - 1 : it0);
-  it1 = it1 < 0 ? 0 : (it1 > nt - 1 ? nt - 1 : it1);
-  ip0 = ((ip0 % np) + np) % np;
-  const long long ip1 = (ip0 + 1) % np;
-  const long long b00 = (ip0 * nt + it0) * nr, b01 = (ip0 * nt + it1) * nr;
-  const long long b10 = (ip1 * nt + it0) * nr, b11 = (ip1 * nt + it1) * nr;
+  // 32-bit index arithmetic (grids are far below 2^31 cells); phi wraps by
+  // an exact floating-point modulo, so any finite phi maps into [0, n_ph).
+  int ir0 = static_cast<int>(fr0);
+  int it0 = static_cast<int>(ft0);
+  const double npd = static_cast<double>(n_ph);
+  int ip0 = static_cast<int>(fp0 - floor(fp0 / npd) * npd);
+  int ir1 = ir0 + 1, it1 = it0 + 1;
+  ir0 = ir0 < 0 ? 0 : (ir0 > n_r - 1 ? n_r - 1 : ir0);
+  ir1 = ir1 < 0 ? 0 : (ir1 > n_r - 1 ? n_r - 1 : ir1);
+  it0 = it0 < 0 ? 0 : (it0 > n_th - 1 ? n_th - 1 : it0);
+  it1 = it1 < 0 ? 0 : (it1 > n_th - 1 ? n_th - 1 : it1);
+  ip0 = ip0 < 0 ? 0 : (ip0 > n_ph - 1 ? n_ph - 1 : ip0);
+  const int ip1 = (ip0 + 1 == n_ph) ? 0 : ip0 + 1;
+  const unsigned b00 = static_cast<unsigned>((ip0 * n_th + it0) * n_r);
+  const unsigned b01 = static_cast<unsigned>((ip0 * n_th + it1) * n_r);
+  const unsigned b10 = static_cast<unsigned>((ip1 * n_th + it0) * n_r);
+  const unsigned b11 = static_cast<unsigned>((ip1 * n_th + it1) * n_r);
   const float4 q000 = cells[b00 + ir0], q001 = cells[b00 + ir1];
   const float4 q010 = cells[b01 + ir0], q011 = cells[b01 + ir1];
   const float4 q100 = cells[b10 + ir0], q101 = cells[b10 + ir1];
   const float4 q110 = cells[b11 + ir0], q111 = cells[b11 + ir1];
-#define BHK_TRI(F_)                                                                          \
+#define BHK_TRI(F_, OUT_)                                                                     \
   {                                                                                         \
     const double c00 = __builtin_fma(fr, (double)q001.F_ - (double)q000.F_, (double)q000.F_); \
     const double c01 = __builtin_fma(fr, (double)q011.F_ - (double)q010.F_, (double)q010.F_); \
@@ -356,12 +364,12 @@ __device__ __forceinline__ void volume_fetch(const float4* __restrict__ cells, c
     const double c11 = __builtin_fma(fr, (double)q111.F_ - (double)q110.F_, (double)q110.F_); \
     const double c0 = __builtin_fma(ft, c01 - c00, c00);                                     \
     const double c1 = __builtin_fma(ft, c11 - c10, c10);                                     \
-    out = __builtin_fma(fp, c1 - c0, c0);                                                    \
+    OUT_ = __builtin_fma(fp, c1 - c0, c0);                                                   \
   }
-  double out;
-  BHK_TRI(x) out3[0] = out;
-  BHK_TRI(y) out3[1] = out;
-  BHK_TRI(z) out3[2] = out;
+  BHK_TRI(x, out4[0])
+  BHK_TRI(y, out4[1])
+  BHK_TRI(z, out4[2])
+  BHK_TRI(w, out4[3])
 #undef BHK_TRI
 }
 
@@ -403,6 +411,69 @@ __device__ __forceinline__ double planck(double nu, double tempK) {
   const double x = (kH / kKB) * nu / tempK;
   if (x > 700.0) return 0.0;
   return (2.0 * kH / (kC * kC)) * nu * nu * nu / expm1(x);
+}
+
+// Emission / absorption per unit affine parameter in the observer's frame
+// at a volume point: nu_f = nu/g, alpha = j/B_nu(T_e) (Kirchhoff,
+// rte_integrator.h:348-357), a = alpha(nu_f) L / g, e = g^2 j(nu_f) L.
+struct VolDev {
+  const float4* cells;
+  int n_r, n_th, n_ph;
+  double r_min, r_max, r_span, sup_r_lo, sup_r_hi, sup_u_max, ell;
+};
+
+__device__ __forceinline__ bool volume_point(const VolDev& v, const RayC& c, double r, double u,
+                                             double phi, double nu_obs, double L, double* a,
+                                             double* e) {
+  *a = 0.0;
+  *e = 0.0;
+  if (!(r >= v.r_min && r <= v.r_max && r >= v.sup_r_lo && r <= v.sup_r_hi &&
+        __builtin_fabs(u) <= v.sup_u_max))
+    return false;
+  double f4[4];
+  volume_fetch(v.cells, v.n_r, v.n_th, v.n_ph, v.r_min, v.r_span, r, u, phi, f4);
+  const double ne = f4[0], th = f4[1], B = f4[2], k2r = f4[3];
+  if (!(ne > 0.0 && th > 0.0 && B > 0.0 && k2r > 0.0)) return true;
+  const double g = fluid_g(r, u, c.a, c.xi, v.ell);
+  if (!(g > 0.0)) return true;
+  const double nu_f = nu_obs / g;
+  const double tempK = th * (kMElectron * kC * kC / kKB);
+  const double Bnu = planck(nu_f, tempK);
+  if (!(Bnu > 0.0)) return true;
+  const double jL = synch_j(nu_f, B, ne, th, k2r) * L;
+  *a = jL / (g * Bnu);
+  *e = g * g * jL;
+  return true;
+}
+
+// d lambda_affine / ds = Sigma / r.
+__device__ __forceinline__ double affine_rate(const RayC& c, double r, double u) {
+  return __builtin_fma(c.a2, u * u, r * r) / r;
+}
+
+// Trapezoidal front-to-back formal solution over one step (second-order
+// form of d_rte_step, src/cuda/device_physics.cuh:1988-2015).
+struct RtAcc {
+  double I, Tr, tau, a0, e0, s0;
+};
+__device__ __forceinline__ void rt_segment(RtAcc& q, double h, double a1, double e1, double s1) {
+  const double asum = q.a0 + a1;
+  const double esum = q.e0 + e1;
+  if (asum > 0.0 || esum > 0.0) {
+    const double dl = 0.5 * h * (q.s0 + s1);
+    if (asum > 0.0) {
+      const double dtau = 0.5 * asum * dl;
+      const double om = -expm1(-dtau);
+      q.I = __builtin_fma(q.Tr * (esum / asum), om, q.I);
+      q.Tr = q.Tr * (1.0 - om);
+      q.tau = q.tau + dtau;
+    } else {
+      q.I = __builtin_fma(q.Tr, 0.5 * esum * dl, q.I);
+    }
+  }
+  q.a0 = a1;
+  q.e0 = e1;
+  q.s0 = s1;
 }
 
 }  // namespace bhk

@@ -35,29 +35,28 @@
 namespace bhk {
 
 constexpr int kBlock = 256;
-constexpr int kMaxK2 = 1024;  // LDS K2 table capacity (doubles)
 
-struct TraceArgs {
+// Launch parameters, resident in device memory next to the camera table
+// (one upload per launch).  The kernel reads them through a pointer rather
+// than as ~350 bytes of kernel arguments, which kept ~100 scalar registers
+// live across the step loop and spilled them.
+struct DevParams {
+  double h_fixed, h_init, h_max, h_vol, rtol, atol;
+  double r_escape, disk_r_out, nu_obs, disk_T_max, length_unit_cm, transmit_min;
+  VolDev vol;
   const CamDev* cams;
-  int n_cams;
-  int batch;
-  long long n_total;
-  unsigned long long* head;  // ray-queue head
-  BH_TraceOptions o;
   const double* luts;
-  int lut_n;
-  BH_Volume vol;
-  const double* k2tab;
-  int k2_n;
-  double k2_lo, k2_hi;
   BH_PixelRecord* out;
+  unsigned long long* head;  // ray-queue head
+  long long n_total;
+  int n_cams, lut_n, batch, max_steps, max_attempts, max_crossings, disk_enabled, pad;
 };
 
 struct Lane {
   RayC c;
   double y[kN], k1[kN];
   double h;
-  double I, Tr, tau;
+  RtAcc rt;
   double cr0, cr1, cr2, cg0, cg1, cg2;
   double r_stop, r_isco;
   long long gid;
@@ -76,18 +75,22 @@ __device__ __forceinline__ int find_camera(const CamDev* cams, int n, long long 
 }
 
 // Returns false for ids in a gap between camera record ranges (no record).
-__device__ __forceinline__ bool lane_setup(Lane& L, const TraceArgs& A) {
-  L.cam = find_camera(A.cams, A.n_cams, L.gid);
-  const CamDev& k = A.cams[L.cam];
+template <bool VOL>
+__device__ __forceinline__ bool lane_setup(Lane& L, const DevParams& P) {
+  L.cam = find_camera(P.cams, P.n_cams, L.gid);
+  const CamDev& k = P.cams[L.cam];
   const long long local = L.gid - k.offset;
   if (local >= k.w * k.h) return false;
   const long long py = local / k.w, px = local - py * k.w;
   L.c = setup_ray(k, px, py, L.y);
   rhs(L.c, L.y, L.k1);
-  L.h = A.o.h_init;
-  L.I = 0.0;
-  L.Tr = 1.0;
-  L.tau = 0.0;
+  L.h = P.h_init;
+  L.rt.I = 0.0;
+  L.rt.Tr = 1.0;
+  L.rt.tau = 0.0;
+  L.rt.a0 = 0.0;
+  L.rt.e0 = 0.0;
+  L.rt.s0 = VOL ? affine_rate(L.c, L.y[0], L.y[1]) : 0.0;
   L.cr0 = L.cr1 = L.cr2 = 0.0;
   L.cg0 = L.cg1 = L.cg2 = 0.0;
   L.r_stop = k.r_stop;
@@ -97,144 +100,116 @@ __device__ __forceinline__ bool lane_setup(Lane& L, const TraceArgs& A) {
   return true;
 }
 
-__device__ __forceinline__ bool in_support(const BH_Volume& v, double r, double u) {
+__device__ __forceinline__ bool in_support(const VolDev& v, double r, double u) {
   return r >= v.sup_r_lo && r <= v.sup_r_hi && __builtin_fabs(u) <= v.sup_u_max;
 }
 
-__device__ __forceinline__ double k2_ratio(const double* k2s, const TraceArgs& A, double th) {
-  const double lt = log(th);
-  return exp(lut_linear(k2s, A.k2_n, A.k2_lo, A.k2_hi, lt));
-}
-
-// One formal-solution segment ending at y (front-to-back), as
-// d_rte_step / d_trace_geodesic_rte (src/cuda/device_physics.cuh:1988-2173)
-// with the thermal-synchrotron coefficients of rte_integrator.h:291-357.
-__device__ __forceinline__ void volume_segment(Lane& L, const TraceArgs& A, const double* k2s,
-                                               const double* y, double h) {
-  const BH_Volume& v = A.vol;
-  const double r = y[0], u = y[1];
-  if (!(r >= v.r_min && r <= v.r_max && in_support(v, r, u))) return;
-  double f3[3];
-  volume_fetch(static_cast<const float4*>(v.cells), v, r, u, y[2], f3);
-  L.nsamp++;
-  const double ne = f3[0], th = f3[1], B = f3[2];
-  if (!(ne > 0.0 && th > 0.0 && B > 0.0)) return;
-  const double g = fluid_g(r, u, L.c.a, L.c.xi, v.ell);
-  if (!(g > 0.0)) return;
-  const double nu_f = A.o.nu_obs / g;
-  const double tempK = th * (kMElectron * kC * kC / kKB);
-  const double Bnu = planck(nu_f, tempK);
-  if (!(Bnu > 0.0)) return;
-  const double j = synch_j(nu_f, B, ne, th, k2_ratio(k2s, A, th));
-  const double alpha_nu = j / Bnu;
-  const double sig = __builtin_fma(L.c.a2, u * u, r * r);
-  const double dl = h * sig / r;
-  const double dtau = alpha_nu * (dl / g) * A.o.length_unit_cm;
-  const double om = -expm1(-dtau);
-  const double g3 = g * g * g;
-  L.I = __builtin_fma(L.Tr * g3 * Bnu, om, L.I);
-  L.Tr = L.Tr * (1.0 - om);
-  L.tau = L.tau + dtau;
+// Volume transfer over the accepted step of length h ending at y.
+__device__ __forceinline__ void volume_segment(Lane& L, const DevParams& P, const double* y,
+                                               double h) {
+  double a1, e1;
+  if (volume_point(P.vol, L.c, y[0], y[1], y[2], P.nu_obs, P.length_unit_cm, &a1, &e1)) L.nsamp++;
+  rt_segment(L.rt, h, a1, e1, affine_rate(L.c, y[0], y[1]));
 }
 
 // Equatorial crossing between (y0, f0) and (y1, f1); true if the disk ends the ray.
-__device__ __forceinline__ bool crossing(Lane& L, const TraceArgs& A, const double* y0,
+__device__ __forceinline__ bool crossing(Lane& L, const DevParams& P, const double* y0,
                                          const double* f0, const double* y1, const double* f1,
                                          double h) {
   if ((y0[1] < 0.0) == (y1[1] < 0.0)) return false;
   const double rc = crossing_radius(y0, f0, y1, f1, h);
   const double g = (rc >= L.r_isco) ? disk_g(rc, L.c.a, L.c.xi) : 0.0;
-  if (L.ncross < A.o.max_crossings) {
-    if (L.ncross == 0) { L.cr0 = rc; L.cg0 = g; }
-    else if (L.ncross == 1) { L.cr1 = rc; L.cg1 = g; }
-    else if (L.ncross == 2) { L.cr2 = rc; L.cg2 = g; }
-  }
+  // Branch-free slot update keeps the six slots in registers (an if/else
+  // chain on ncross is lowered to a dynamically indexed scratch array).
+  const bool rec_ok = L.ncross < P.max_crossings;
+  const bool s0 = rec_ok && L.ncross == 0, s1 = rec_ok && L.ncross == 1, s2 = rec_ok && L.ncross == 2;
+  L.cr0 = s0 ? rc : L.cr0;
+  L.cg0 = s0 ? g : L.cg0;
+  L.cr1 = s1 ? rc : L.cr1;
+  L.cg1 = s1 ? g : L.cg1;
+  L.cr2 = s2 ? rc : L.cr2;
+  L.cg2 = s2 ? g : L.cg2;
   if (L.ncross < 255) L.ncross++;
-  if (A.o.disk_enabled && rc >= L.r_isco && rc <= A.o.disk_r_out) {
-    const double* lut = A.luts + static_cast<long long>(A.cams[L.cam].lut_index) * A.lut_n;
-    const double F = lut_linear(lut, A.lut_n, L.r_isco, A.o.disk_r_out, rc);
-    L.I = __builtin_fma(L.Tr, disk_intensity(F, g, A.o.nu_obs, A.o.disk_T_max), L.I);
+  if (P.disk_enabled && rc >= L.r_isco && rc <= P.disk_r_out) {
+    const double* lut = P.luts + static_cast<long long>(P.cams[L.cam].lut_index) * P.lut_n;
+    const double F = lut_linear(lut, P.lut_n, L.r_isco, P.disk_r_out, rc);
+    L.rt.I = __builtin_fma(L.rt.Tr, disk_intensity(F, g, P.nu_obs, P.disk_T_max), L.rt.I);
     return true;
   }
   return false;
 }
 
-__device__ __forceinline__ int terminate_check(const Lane& L, const TraceArgs& A) {
+__device__ __forceinline__ int terminate_check(const Lane& L, const DevParams& P) {
   if (L.y[0] < L.r_stop) return BH_HIT_HORIZON;
-  if (L.y[0] > A.o.r_escape) return BH_HIT_ESCAPED;
-  if (A.o.transmit_min > 0.0 && L.Tr < A.o.transmit_min) return BH_HIT_OPAQUE;
+  if (L.y[0] > P.r_escape) return BH_HIT_ESCAPED;
+  if (P.transmit_min > 0.0 && L.rt.Tr < P.transmit_min) return BH_HIT_OPAQUE;
   return 0;
 }
 
-__device__ __forceinline__ void write_record(const Lane& L, const TraceArgs& A, int status) {
-  BH_PixelRecord rec;
-  rec.intensity = L.I;
-  rec.r_exit = L.y[0];
-  rec.u_exit = L.y[1];
-  rec.phi_exit = L.y[2];
-  rec.cross_r[0] = L.cr0;
-  rec.cross_r[1] = L.cr1;
-  rec.cross_r[2] = L.cr2;
-  rec.cross_g[0] = L.cg0;
-  rec.cross_g[1] = L.cg1;
-  rec.cross_g[2] = L.cg2;
-  rec.tau = L.tau;
+// Field-by-field stores straight into the caller's record (a local record
+// struct would be staged through scratch).
+__device__ __forceinline__ void write_record(const Lane& L, const DevParams& P, int status) {
+  BH_PixelRecord* __restrict__ rec = P.out + L.gid;
   const double R = radial_potential(L.c, L.y[0]);
   const double U = polar_potential(L.c, L.y[1]);
   const double rs = L.y[0] * L.y[0] + L.c.a2;
-  rec.drift_null = __builtin_fabs(__builtin_fma(L.y[3], L.y[3], -R)) / (rs * rs);
-  rec.drift_carter = __builtin_fabs(__builtin_fma(L.y[4], L.y[4], -U)) /
-                     (__builtin_fabs(L.c.eta) + L.c.a2 + L.c.xi * L.c.xi);
-  rec.steps = L.steps;
-  rec.attempts = L.attempts;
-  rec.hit_class = static_cast<uint8_t>(status);
-  rec.n_cross = static_cast<uint8_t>(L.ncross);
-  rec.flags = 0;
-  rec.pad0 = 0;
-  rec.confidence = 1.0f;
-  rec.fp64_steps = L.steps;
-  rec.n_samples = L.nsamp;
-  A.out[L.gid] = rec;
+  double2* d2 = reinterpret_cast<double2*>(rec);
+  d2[0] = make_double2(L.rt.I, L.y[0]);
+  d2[1] = make_double2(L.y[1], L.y[2]);
+  d2[2] = make_double2(L.cr0, L.cr1);
+  d2[3] = make_double2(L.cr2, L.cg0);
+  d2[4] = make_double2(L.cg1, L.cg2);
+  d2[5] = make_double2(L.rt.tau, __builtin_fabs(__builtin_fma(L.y[3], L.y[3], -R)) / (rs * rs));
+  const double dc = __builtin_fabs(__builtin_fma(L.y[4], L.y[4], -U)) /
+                    (__builtin_fabs(L.c.eta) + L.c.a2 + L.c.xi * L.c.xi);
+  int* i4 = reinterpret_cast<int*>(rec);
+  reinterpret_cast<double*>(rec)[12] = dc;
+  i4[26] = L.steps;
+  i4[27] = L.attempts;
+  i4[28] = status | (L.ncross << 8);  // hit_class, n_cross, flags = 0, pad0 = 0
+  i4[29] = __float_as_int(1.0f);      // confidence
+  i4[30] = L.steps;                   // fp64_steps
+  i4[31] = L.nsamp;
 }
 
 // One RK4 step; returns the terminal status (0 = continue).
 template <bool VOL>
-__device__ __forceinline__ int step_rk4(Lane& L, const TraceArgs& A, const double* k2s) {
-  if (L.steps >= A.o.max_steps) return BH_HIT_MAX_STEPS;
-  const double h = A.o.h_fixed;
+__device__ __forceinline__ int step_rk4(Lane& L, const DevParams& P) {
+  if (L.steps >= P.max_steps) return BH_HIT_MAX_STEPS;
+  const double h = P.h_fixed;
   double yn[kN], fn[kN];
   rk4_step(L.c, L.y, L.k1, h, yn);
   rhs(L.c, yn, fn);
   project_radial(L.c, yn, fn);
   L.steps++;
   L.attempts++;
-  if (VOL) volume_segment(L, A, k2s, yn, h);
-  const bool hit = crossing(L, A, L.y, L.k1, yn, fn, h);
+  if (VOL) volume_segment(L, P, yn, h);
+  const bool hit = crossing(L, P, L.y, L.k1, yn, fn, h);
 #pragma unroll
   for (int i = 0; i < kN; ++i) {
     L.y[i] = yn[i];
     L.k1[i] = fn[i];
   }
   if (hit) return BH_HIT_DISK;
-  return terminate_check(L, A);
+  return terminate_check(L, P);
 }
 
 // One DP45 attempt; returns the terminal status (0 = continue).
 template <bool VOL>
-__device__ __forceinline__ int step_dp45(Lane& L, const TraceArgs& A, const double* k2s) {
-  if (L.steps >= A.o.max_steps || L.attempts >= A.o.max_attempts) return BH_HIT_MAX_STEPS;
-  double cap = A.o.h_max;
-  if (VOL && in_support(A.vol, L.y[0], L.y[1]) && A.o.h_vol < cap) cap = A.o.h_vol;
+__device__ __forceinline__ int step_dp45(Lane& L, const DevParams& P) {
+  if (L.steps >= P.max_steps || L.attempts >= P.max_attempts) return BH_HIT_MAX_STEPS;
+  double cap = P.h_max;
+  if (VOL && in_support(P.vol, L.y[0], L.y[1]) && P.h_vol < cap) cap = P.h_vol;
   if (L.h > cap) L.h = cap;
   L.attempts++;
   double y5[kN], k7[kN];
-  const double err = dp45_attempt(L.c, L.y, L.k1, L.h, A.o.rtol, A.o.atol, y5, k7);
+  const double err = dp45_attempt(L.c, L.y, L.k1, L.h, P.rtol, P.atol, y5, k7);
   if (err <= 1.0) {
     const double fac = dp45_factor(err, true, L.last_rej);
     project_radial(L.c, y5, k7);
     L.steps++;
-    if (VOL) volume_segment(L, A, k2s, y5, L.h);
-    const bool hit = crossing(L, A, L.y, L.k1, y5, k7, L.h);
+    if (VOL) volume_segment(L, P, y5, L.h);
+    const bool hit = crossing(L, P, L.y, L.k1, y5, k7, L.h);
 #pragma unroll
     for (int i = 0; i < kN; ++i) {
       L.y[i] = y5[i];
@@ -243,7 +218,7 @@ __device__ __forceinline__ int step_dp45(Lane& L, const TraceArgs& A, const doub
     L.h = L.h * fac;
     L.last_rej = false;
     if (hit) return BH_HIT_DISK;
-    return terminate_check(L, A);
+    return terminate_check(L, P);
   }
   L.h = L.h * dp45_factor(err, false, L.last_rej);
   L.last_rej = true;
@@ -251,12 +226,8 @@ __device__ __forceinline__ int step_dp45(Lane& L, const TraceArgs& A, const doub
 }
 
 template <int INTEG, bool VOL>
-__global__ void __launch_bounds__(kBlock) k_trace_fp64(TraceArgs A) {
-  __shared__ double k2s[kMaxK2];
-  if (VOL) {
-    for (int i = threadIdx.x; i < A.k2_n; i += kBlock) k2s[i] = A.k2tab[i];
-    __syncthreads();
-  }
+__global__ void __launch_bounds__(kBlock) k_trace_fp64(const DevParams* __restrict__ Pp) {
+  const DevParams& P = *Pp;
   const int lane = static_cast<int>(__lane_id());
   const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   Lane L;
@@ -268,15 +239,15 @@ __global__ void __launch_bounds__(kBlock) k_trace_fp64(TraceArgs A) {
     if (m) {
       const int leader = __ffsll(static_cast<long long>(m)) - 1;
       unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(A.head, static_cast<unsigned long long>(__popcll(m)));
+      if (lane == leader) base = atomicAdd(P.head, static_cast<unsigned long long>(__popcll(m)));
       const unsigned int lo = __shfl(static_cast<unsigned int>(base & 0xffffffffull), leader);
       const unsigned int hi = __shfl(static_cast<unsigned int>(base >> 32), leader);
       base = (static_cast<unsigned long long>(hi) << 32) | lo;
       if (need) {
         const long long id = static_cast<long long>(base) + __popcll(m & below);
-        if (id < A.n_total) {
+        if (id < P.n_total) {
           L.gid = id;
-          if (!lane_setup(L, A)) L.gid = -1;
+          if (!lane_setup<VOL>(L, P)) L.gid = -1;
         } else {
           done = true;
         }
@@ -284,11 +255,10 @@ __global__ void __launch_bounds__(kBlock) k_trace_fp64(TraceArgs A) {
     }
     if (__ballot(!done) == 0ull) break;
     if (L.gid >= 0) {
-      for (int b = 0; b < A.batch; ++b) {
-        const int st = (INTEG == BH_INTEGRATOR_DP45) ? step_dp45<VOL>(L, A, k2s)
-                                                      : step_rk4<VOL>(L, A, k2s);
+      for (int b = 0; b < P.batch; ++b) {
+        const int st = (INTEG == BH_INTEGRATOR_DP45) ? step_dp45<VOL>(L, P) : step_rk4<VOL>(L, P);
         if (st) {
-          write_record(L, A, st);
+          write_record(L, P, st);
           L.gid = -1;
           break;
         }
@@ -315,9 +285,12 @@ int fail(const std::string& msg) {
   return -1;
 }
 
+// Per (device, stream) launch context: one device block holding the
+// DevParams followed by the camera table, and the ray-queue head.  Reused
+// across launches on the same stream (stream order serialises them).
 struct StreamCtx {
-  CamDev* d_cams = nullptr;
-  size_t cam_cap = 0;
+  unsigned char* d_block = nullptr;
+  size_t cap = 0;
   unsigned long long* d_head = nullptr;
 };
 std::mutex g_mu;
@@ -340,10 +313,10 @@ int occupancy_grid() {
 }
 
 template <int I, bool V>
-hipError_t launch(const TraceArgs& A, hipStream_t s, int grid_override) {
+hipError_t launch(const DevParams* d_params, hipStream_t s, int grid_override) {
   const int grid = grid_override > 0 ? grid_override : occupancy_grid<I, V>();
   if (grid <= 0) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((k_trace_fp64<I, V>), dim3(grid), dim3(kBlock), 0, s, A);
+  hipLaunchKernelGGL((k_trace_fp64<I, V>), dim3(grid), dim3(kBlock), 0, s, d_params);
   return hipGetLastError();
 }
 
@@ -396,7 +369,7 @@ double bh_isco_prograde(double a) {
 
 int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
                          const double* disk_luts, int disk_lut_n, const BH_Volume* vol,
-                         const BH_K2Table* k2, BH_PixelRecord* out, void* stream) {
+                         BH_PixelRecord* out, void* stream) {
   if (!cams || n_cams <= 0) return fail("bh_trace_image_plane: no cameras");
   if (!opts) return fail("bh_trace_image_plane: NULL options");
   if (!out) return fail("bh_trace_image_plane: NULL output");
@@ -412,6 +385,7 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
     return fail("bh_trace_image_plane: DP45 needs h_init, h_max, rtol, atol > 0");
   if (o.max_crossings < 0 || o.max_crossings > 3)
     return fail("bh_trace_image_plane: max_crossings must be in [0, 3]");
+  if (o.max_steps < 0 || o.max_attempts < 0) return fail("bh_trace_image_plane: negative budget");
   if (o.disk_enabled && (!disk_luts || disk_lut_n < 2))
     return fail("bh_trace_image_plane: disk enabled without disk LUTs");
   const bool vol_on = o.volume_enabled != 0;
@@ -419,10 +393,10 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
     if (!vol || !vol->cells || vol->n_r < 1 || vol->n_th < 1 || vol->n_ph < 1 ||
         !(vol->r_max > vol->r_min))
       return fail("bh_trace_image_plane: volume enabled with an invalid BH_Volume");
+    if (static_cast<long long>(vol->n_r) * vol->n_th * vol->n_ph >= (1ll << 31))
+      return fail("bh_trace_image_plane: volume grid too large (>= 2^31 cells)");
     if (o.integrator == BH_INTEGRATOR_DP45 && !(o.h_vol > 0.0))
       return fail("bh_trace_image_plane: h_vol must be > 0");
-    if (!k2 || !k2->tab || k2->n < 2 || k2->n > kMaxK2 || !(k2->lnth_max > k2->lnth_min))
-      return fail("bh_trace_image_plane: volume enabled with an invalid BH_K2Table");
   }
   std::vector<CamDev> hc(n_cams);
   long long total = 0;
@@ -432,11 +406,13 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
     if (!(c.theta_obs > 0.0 && c.theta_obs < kPi))
       return fail("bh_trace_image_plane: theta_obs outside (0, pi)");
     if (c.width <= 0 || c.height <= 0) return fail("bh_trace_image_plane: empty camera");
-    if (i > 0 && c.record_offset < cams[i - 1].record_offset + static_cast<long long>(cams[i - 1].width) * cams[i - 1].height)
-      return fail("bh_trace_image_plane: camera record ranges must be increasing and disjoint");
     if (c.record_offset < 0) return fail("bh_trace_image_plane: negative record_offset");
+    if (i > 0 && c.record_offset < cams[i - 1].record_offset +
+                                       static_cast<long long>(cams[i - 1].width) * cams[i - 1].height)
+      return fail("bh_trace_image_plane: camera record ranges must be increasing and disjoint");
     const double rh = bh_outer_horizon(c.spin);
-    if (!(c.r_obs > o.horizon_factor * rh)) return fail("bh_trace_image_plane: r_obs inside the horizon stop");
+    if (!(c.r_obs > o.horizon_factor * rh))
+      return fail("bh_trace_image_plane: r_obs inside the horizon stop");
     CamDev& k = hc[i];
     k.a = c.spin;
     k.sin_o = std::sin(c.theta_obs);
@@ -457,46 +433,68 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
   hipStream_t s = static_cast<hipStream_t>(stream);
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return fail("bh_trace_image_plane: no HIP device");
+  const size_t bytes = sizeof(DevParams) + hc.size() * sizeof(CamDev);
   StreamCtx* ctx;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     ctx = &g_ctx[{dev, stream}];
-    if (ctx->cam_cap < hc.size()) {
-      if (ctx->d_cams) (void)hipFree(ctx->d_cams);
-      ctx->d_cams = nullptr;
-      ctx->cam_cap = 0;
-      const size_t cap = std::max<size_t>(hc.size(), 64);
-      if (hipMalloc(&ctx->d_cams, cap * sizeof(CamDev)) != hipSuccess)
-        return fail("bh_trace_image_plane: hipMalloc cameras failed");
-      ctx->cam_cap = cap;
+    if (ctx->cap < bytes) {
+      if (ctx->d_block) (void)hipFree(ctx->d_block);
+      ctx->d_block = nullptr;
+      ctx->cap = 0;
+      const size_t cap = std::max<size_t>(bytes, sizeof(DevParams) + 64 * sizeof(CamDev));
+      if (hipMalloc(&ctx->d_block, cap) != hipSuccess)
+        return fail("bh_trace_image_plane: hipMalloc of the launch block failed");
+      ctx->cap = cap;
     }
     if (!ctx->d_head && hipMalloc(&ctx->d_head, 256) != hipSuccess)
-      return fail("bh_trace_image_plane: hipMalloc queue head failed");
+      return fail("bh_trace_image_plane: hipMalloc of the queue head failed");
   }
-  if (hipMemcpyAsync(ctx->d_cams, hc.data(), hc.size() * sizeof(CamDev), hipMemcpyHostToDevice, s) !=
-      hipSuccess)
-    return fail("bh_trace_image_plane: camera upload failed");
-  TraceArgs A;
-  A.cams = ctx->d_cams;
-  A.n_cams = n_cams;
-  A.batch = o.refill_batch > 0 ? o.refill_batch : (o.integrator == BH_INTEGRATOR_DP45 ? 8 : 16);
-  A.n_total = total;
-  A.head = ctx->d_head;
-  A.o = o;
-  A.luts = disk_luts;
-  A.lut_n = disk_lut_n;
-  std::memset(&A.vol, 0, sizeof(A.vol));
-  A.k2tab = nullptr;
-  A.k2_n = 0;
-  A.k2_lo = A.k2_hi = 0.0;
+  std::vector<unsigned char> blk(bytes);
+  DevParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.h_fixed = o.h_fixed;
+  P.h_init = o.h_init;
+  P.h_max = o.h_max;
+  P.h_vol = o.h_vol;
+  P.rtol = o.rtol;
+  P.atol = o.atol;
+  P.r_escape = o.r_escape;
+  P.disk_r_out = o.disk_r_out;
+  P.nu_obs = o.nu_obs;
+  P.disk_T_max = o.disk_T_max;
+  P.length_unit_cm = o.length_unit_cm;
+  P.transmit_min = o.transmit_min;
   if (vol_on) {
-    A.vol = *vol;
-    A.k2tab = k2->tab;
-    A.k2_n = k2->n;
-    A.k2_lo = k2->lnth_min;
-    A.k2_hi = k2->lnth_max;
+    P.vol.cells = static_cast<const float4*>(vol->cells);
+    P.vol.n_r = vol->n_r;
+    P.vol.n_th = vol->n_th;
+    P.vol.n_ph = vol->n_ph;
+    P.vol.r_min = vol->r_min;
+    P.vol.r_max = vol->r_max;
+    P.vol.r_span = vol->r_max - vol->r_min;
+    P.vol.sup_r_lo = vol->sup_r_lo;
+    P.vol.sup_r_hi = vol->sup_r_hi;
+    P.vol.sup_u_max = vol->sup_u_max;
+    P.vol.ell = vol->ell;
   }
-  A.out = out;
+  P.cams = reinterpret_cast<const CamDev*>(ctx->d_block + sizeof(DevParams));
+  P.luts = disk_luts;
+  P.out = out;
+  P.head = ctx->d_head;
+  P.n_total = total;
+  P.n_cams = n_cams;
+  P.lut_n = disk_lut_n;
+  P.batch = o.refill_batch > 0 ? o.refill_batch : (o.integrator == BH_INTEGRATOR_DP45 ? 8 : 16);
+  P.max_steps = o.max_steps;
+  P.max_attempts = o.max_attempts;
+  P.max_crossings = o.max_crossings;
+  P.disk_enabled = o.disk_enabled;
+  std::memcpy(blk.data(), &P, sizeof(P));
+  std::memcpy(blk.data() + sizeof(DevParams), hc.data(), hc.size() * sizeof(CamDev));
+  // Pageable source: the runtime stages it before returning, so blk may die.
+  if (hipMemcpyAsync(ctx->d_block, blk.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail("bh_trace_image_plane: parameter upload failed");
   if (!g_have_events) {
     if (hipEventCreate(&g_ev0) != hipSuccess || hipEventCreate(&g_ev1) != hipSuccess)
       return fail("bh_trace_image_plane: hipEventCreate failed");
@@ -505,12 +503,14 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
   hipLaunchKernelGGL(k_reset_head, dim3(1), dim3(64), 0, s, ctx->d_head);
   if (hipGetLastError() != hipSuccess) return fail("bh_trace_image_plane: reset launch failed");
   (void)hipEventRecord(g_ev0, s);
+  const DevParams* dp = reinterpret_cast<const DevParams*>(ctx->d_block);
   hipError_t e;
   if (o.integrator == BH_INTEGRATOR_DP45)
-    e = vol_on ? launch<BH_INTEGRATOR_DP45, true>(A, s, 0) : launch<BH_INTEGRATOR_DP45, false>(A, s, 0);
+    e = vol_on ? launch<BH_INTEGRATOR_DP45, true>(dp, s, 0) : launch<BH_INTEGRATOR_DP45, false>(dp, s, 0);
   else
-    e = vol_on ? launch<BH_INTEGRATOR_RK4, true>(A, s, 0) : launch<BH_INTEGRATOR_RK4, false>(A, s, 0);
-  if (e != hipSuccess) return fail(std::string("bh_trace_image_plane: launch failed: ") + hipGetErrorString(e));
+    e = vol_on ? launch<BH_INTEGRATOR_RK4, true>(dp, s, 0) : launch<BH_INTEGRATOR_RK4, false>(dp, s, 0);
+  if (e != hipSuccess)
+    return fail(std::string("bh_trace_image_plane: launch failed: ") + hipGetErrorString(e));
   (void)hipEventRecord(g_ev1, s);
   return 0;
 }

@@ -11,10 +11,11 @@ DP45_ATTEMPT = 521      # 6 RHS (FSAL) + stage sums + error norm + controller
 PROJECTION = 14         # radial null-constraint projection (accepted DP45 steps)
 CROSSING = 161          # Hermite root of u(s) = 0 + disk g
 DISK_HIT = 14           # LUT lookup + observed blackbody (+1 transcendental)
-VOLUME_SAMPLE = 166     # trilinear fetch + fluid g + Planck + K2 + j + segment (+7 transcendentals)
+VOLUME_SAMPLE = 186     # trilinear fetch (4 fields) + fluid g + Planck + j + a, e + trapezoid segment (+5 transcendentals)
+VOLUME_STEP_NOSAMPLE = 7  # affine rate + segment bookkeeping on accepted steps without a sample
 SETUP = 65              # image plane -> constants + first RHS
 
-VOLUME_SAMPLE_BYTES = 96  # algorithmic bytes per trilinear sample: 8 corners x 3 FP32 fields
+VOLUME_SAMPLE_BYTES = 128  # algorithmic bytes per trilinear sample: 8 corners x 4 FP32 fields
 
 
 def frame_flops(rec, integrator: int) -> float:
@@ -30,4 +31,7 @@ def frame_flops(rec, integrator: int) -> float:
         f = attempts * DP45_ATTEMPT + steps * PROJECTION
     else:
         f = steps * RK4_STEP
-    return f + n * SETUP + ncross * CROSSING + disk * DISK_HIT + samples * VOLUME_SAMPLE
+    vol = 0.0
+    if samples > 0:
+        vol = samples * VOLUME_SAMPLE + (steps - samples) * VOLUME_STEP_NOSAMPLE
+    return f + n * SETUP + ncross * CROSSING + disk * DISK_HIT + vol

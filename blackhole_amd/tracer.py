@@ -19,10 +19,6 @@ import numpy as np
 from . import abi
 from ._lib import check, lib
 
-K2_N = 1024
-K2_LNTH = (math.log(0.05), math.log(2000.0))
-
-
 def default_options(**overrides) -> abi.BH_TraceOptions:
     o = abi.BH_TraceOptions()
     check(lib().bh_trace_options_default(C.byref(o)), "bh_trace_options_default")
@@ -68,13 +64,6 @@ def disk_luts(cams: Sequence[abi.BH_Camera], r_out: float, n: int = 256) -> np.n
     return out
 
 
-def k2_table(n: int = K2_N, lo: float = K2_LNTH[0], hi: float = K2_LNTH[1]) -> np.ndarray:
-    out = np.zeros(n)
-    check(lib().bh_k2_table_build(n, lo, hi, out.ctypes.data_as(C.POINTER(C.c_double))),
-          "bh_k2_table_build")
-    return out
-
-
 def torus_params(spin: float, n_r: int = 256, n_th: int = 128, n_ph: int = 64,
                  r_hi: float = 50.0, seed: int = 0x5EED0001, **kw) -> abi.BH_TorusParams:
     p = abi.BH_TorusParams()
@@ -97,7 +86,8 @@ def torus_params(spin: float, n_r: int = 256, n_th: int = 128, n_ph: int = 64,
 
 
 def generate_torus(p: abi.BH_TorusParams):
-    """Host torus grid: returns (cells float32[n_ph, n_th, n_r, 4], BH_Volume sans pointer)."""
+    """Host torus grid: returns (cells float32[n_ph, n_th, n_r, 4] holding n_e, Theta_e,
+    |B| and K_2(1/Theta_e)/(2 Theta_e^2), BH_Volume without the device pointer)."""
     cells = np.zeros((p.n_ph, p.n_th, p.n_r, 4), dtype=np.float32)
     vol = abi.BH_Volume()
     check(lib().bh_volume_generate_torus(C.byref(p), cells.ctypes.data_as(C.POINTER(C.c_float)),
@@ -115,8 +105,6 @@ class DeviceScene:
     lut_n: int = 0
     cells: Optional[object] = None     # torch float32 [.., 4] on cuda
     vol: Optional[abi.BH_Volume] = None
-    k2tab: Optional[object] = None     # torch float64 [n] on cuda
-    k2: Optional[abi.BH_K2Table] = None
 
 
 def _torch():
@@ -127,8 +115,7 @@ def _torch():
 
 
 def upload_scene(cams, opts, luts: Optional[np.ndarray] = None, cells: Optional[np.ndarray] = None,
-                 vol: Optional[abi.BH_Volume] = None, k2tab: Optional[np.ndarray] = None,
-                 device: str = "cuda:0") -> DeviceScene:
+                 vol: Optional[abi.BH_Volume] = None, device: str = "cuda:0") -> DeviceScene:
     torch = _torch()
     n = max(c.record_offset + c.width * c.height for c in cams)
     sc = DeviceScene(cams=list(cams), opts=opts, n_records=n)
@@ -140,9 +127,6 @@ def upload_scene(cams, opts, luts: Optional[np.ndarray] = None, cells: Optional[
         sc.vol = abi.BH_Volume()
         C.pointer(sc.vol)[0] = vol
         sc.vol.cells = sc.cells.data_ptr()
-    if k2tab is not None:
-        sc.k2tab = torch.from_numpy(np.ascontiguousarray(k2tab, dtype=np.float64)).to(device)
-        sc.k2 = abi.BH_K2Table(sc.k2tab.data_ptr(), k2tab.size, 0, K2_LNTH[0], K2_LNTH[1])
     return sc
 
 
@@ -162,7 +146,6 @@ def launch(sc: DeviceScene, out, stream=None) -> None:
         cam_arr, len(sc.cams), C.byref(sc.opts),
         C.c_void_p(sc.luts.data_ptr()) if sc.luts is not None else None, sc.lut_n,
         C.byref(sc.vol) if sc.vol is not None else None,
-        C.byref(sc.k2) if sc.k2 is not None else None,
         C.c_void_p(out.data_ptr()), C.c_void_p(s.cuda_stream),
     )
     check(rc, "bh_trace_image_plane")

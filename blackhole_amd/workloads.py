@@ -20,7 +20,7 @@ import math
 import numpy as np
 
 from . import abi
-from .tracer import camera, default_options, disk_luts, generate_torus, k2_table, pack_cameras, torus_params
+from .tracer import camera, default_options, disk_luts, generate_torus, pack_cameras, torus_params
 
 KEV_HZ = 2.417989242e17
 
@@ -31,7 +31,7 @@ def c1(width: int = 512, height: int = 512):
         integrator=abi.INTEGRATOR_RK4, precision=abi.PRECISION_FP64, max_steps=2000,
         disk_enabled=1, volume_enabled=0, h_fixed=0.01, horizon_factor=1.01, r_escape=200.0,
         disk_r_out=20.0, nu_obs=KEV_HZ, disk_T_max=1.0e7)
-    return dict(cams=cams, opts=opts, luts=disk_luts(cams, 20.0), cells=None, vol=None, k2tab=None)
+    return dict(cams=cams, opts=opts, luts=disk_luts(cams, 20.0), cells=None, vol=None)
 
 
 def c2(width: int = 1024, height: int = 1024, grid=(256, 128, 64), seed: int = 0x5EED0002):
@@ -43,7 +43,7 @@ def c2(width: int = 1024, height: int = 1024, grid=(256, 128, 64), seed: int = 0
         nu_obs=230.0e9)
     tp = torus_params(0.94, n_r=grid[0], n_th=grid[1], n_ph=grid[2], seed=seed)
     cells, vol = generate_torus(tp)
-    return dict(cams=cams, opts=opts, luts=None, cells=cells, vol=vol, k2tab=k2_table(), torus=tp)
+    return dict(cams=cams, opts=opts, luts=None, cells=cells, vol=vol, torus=tp)
 
 
 def c4_cameras(n: int = 256, seed: int = 0x5EED0004):

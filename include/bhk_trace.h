@@ -122,7 +122,8 @@ typedef struct BH_TraceOptions {
 
 /* Synthetic GRMHD-like volume on a Boyer-Lindquist (r, theta, phi) grid.
  * cells: device pointer to n_r*n_th*n_ph float4 {n_e [cm^-3], Theta_e
- * (k T_e / m_e c^2), |B| [G], 0}; index ((iph * n_th) + ith) * n_r + ir.
+ * (k T_e / m_e c^2), |B| [G], K_2(1/Theta_e) / (2 Theta_e^2)};
+ * index ((iph * n_th) + ith) * n_r + ir; fewer than 2^31 cells.
  * Cell-centred samples: cell i of an axis of n cells spans
  * [lo + i*d, lo + (i+1)*d]; r clamps, theta clamps (mirror at the poles is
  * the same for the half-cell overhang), phi wraps (2*pi periodic) -- the
@@ -136,15 +137,6 @@ typedef struct BH_Volume {
     double  ell;            /* constant specific angular momentum -u_phi/u_t of
                                the fluid (rotation law of the torus)            */
 } BH_Volume;
-
-/* Thermal-synchrotron K_2 table: tab[i] = ln(K_2(1/Theta) / (2 Theta^2)) at
- * ln Theta = lnth_min + i * (lnth_max - lnth_min) / (n - 1); linear
- * interpolation, clamped outside. */
-typedef struct BH_K2Table {
-    const double* tab;      /* device double[n]                                 */
-    int32_t n, pad0;
-    double  lnth_min, lnth_max;
-} BH_K2Table;
 
 /* ---------------------------------------------------------------------- */
 /* Output                                                                  */
@@ -198,9 +190,6 @@ double bh_isco_prograde(double spin);
  * (src/physics/lut.h:50-86, src/physics/thin_disk.h:243-266 with M = 1). */
 int bh_disk_lut_build(double spin, double r_in, double r_out, int n, double* out);
 
-/* Host: the K_2 table of BH_K2Table (n entries over [lnth_min, lnth_max]). */
-int bh_k2_table_build(int n, double lnth_min, double lnth_max, double* out);
-
 /* Host: synthetic torus volume (Fishbone & Moncrief 1976 constant-l torus,
  * modulated by a seeded lognormal Gaussian random field with a k^(-11/3)
  * spectrum).  cells_host: float[4 * n_r*n_th*n_ph]; fills vol's scalar fields
@@ -221,16 +210,18 @@ typedef struct BH_TorusParams {
 int bh_volume_generate_torus(const BH_TorusParams* p, float* cells_host, BH_Volume* vol);
 
 /* Trace every pixel of n_cameras cameras into out (device array of
- * BH_PixelRecord, caller-owned, sized by the cameras' record ranges).
- * cams and opts are host pointers.  disk_luts: device double array holding
- * n_cameras tables of disk_lut_n entries (camera c uses
- * disk_luts + c*disk_lut_n, spanning [r_isco(spin_c), opts->disk_r_out]);
- * may be NULL when the disk is disabled.  vol / k2 may be NULL when the
- * volume is disabled.  Asynchronous on stream. */
+ * BH_PixelRecord, caller-owned, sized by the cameras' record ranges; records
+ * in gaps between ranges are left untouched).  cams and opts are host
+ * pointers.  disk_luts: device double array holding n_cameras tables of
+ * disk_lut_n entries (camera c uses disk_luts + c*disk_lut_n, spanning
+ * [r_isco(spin_c), opts->disk_r_out]); may be NULL when the disk is
+ * disabled.  vol may be NULL when the volume is disabled.  Asynchronous on
+ * stream; launches on one stream are serialised, launches on different
+ * streams are independent. */
 int bh_trace_image_plane(const BH_Camera* cams, int n_cameras,
                          const BH_TraceOptions* opts,
                          const double* disk_luts, int disk_lut_n,
-                         const BH_Volume* vol, const BH_K2Table* k2,
+                         const BH_Volume* vol,
                          BH_PixelRecord* out, void* stream);
 
 /* Elapsed time of the last bh_trace_image_plane kernel on this thread, from

@@ -115,36 +115,26 @@ int main() {
   disk_intensity(F(0.5), F(0.9), 2.418e17, 1e7);
   const Tally t_disk = take();
 
-  // Volume sample: trilinear fetch (3 fields) + fluid g + Planck + K2 table +
-  // synchrotron j + the formal-solution segment (as volume_segment).
+  // Volume sample: trilinear fetch (4 fields) + fluid g + Planck + synchrotron
+  // j + observer-frame a, e (volume_point), the affine rate and the
+  // trapezoidal segment (rt_segment), as the trace loop runs them per sample.
   static float cells[4 * 8 * 8 * 8];
   for (int i = 0; i < 4 * 512; ++i) cells[i] = 1.0f;
   VolumeHost V;
   V.cells = cells;
   V.v = BH_Volume{nullptr, 8, 8, 8, 0, 1.0, 50.0, 2.0, 40.0, 0.8, 4.0};
-  static double k2tab[64];
-  K2Host K;
-  K.tab = k2tab;
-  K.t = BH_K2Table{nullptr, 64, 0, -2.0, 7.0};
   cnt::reset();
-  F f3[3];
-  volume_fetch(V, F(8.0), F(0.1), F(1.0), f3);
-  const F g = fluid_g(F(8.0), F(0.1), a, xi, F(3.5));
-  const F nu_f = F(2.3e11) / g;
-  const F tempK = F(10.0) * F(kMElectron * kC * kC / kKB);
-  const F Bnu = planck(nu_f, tempK);
-  const F j = synch_j(nu_f, F(30.0), F(1e6), F(10.0), k2_ratio(K, F(10.0)));
-  const F alpha_nu = j / Bnu;
-  const F sig = fma_(c.a2, F(0.01), F(64.0));
-  const F dl = F(0.01) * sig / F(8.0);
-  const F dtau = alpha_nu * (dl / g) * F(5.9e11);
-  const F om = -expm1_(-dtau);
-  const F g3 = g * g * g;
-  F I(0.0), Tr(1.0), tau(0.0);
-  I = fma_(Tr * g3 * Bnu, om, I);
-  Tr = Tr * (F(1.0) - om);
-  tau = tau + dtau;
+  F a1, e1;
+  volume_point(V, c, F(8.0), F(0.1), F(1.0), 2.3e11, 5.9e11, &a1, &e1);
+  RtAcc<F> q{F(0.0), F(1.0), F(0.0), F(1e-3), F(1e-20), F(9.0)};
+  rt_segment(q, F(0.01), a1, e1, affine_rate(c, F(8.0), F(0.1)));
   const Tally t_vol = take();
+
+  // Volume bookkeeping outside the support: only the affine rate and the
+  // segment shift (no sample).
+  RtAcc<F> q0{F(0.0), F(1.0), F(0.0), F(0.0), F(0.0), F(9.0)};
+  rt_segment(q0, F(0.01), F(0.0), F(0.0), affine_rate(c, F(60.0), F(0.1)));
+  const Tally t_vol_skip = take();
 
   // Ray setup (Bardeen mapping + R(r_obs) + first RHS).
   CamConst kc{0.94, 0.29, 0.95, 100.0, 1.4, 2.0, -15.0, 0.03, 15.0, 0.03, 1024, 1024, 0};
@@ -156,10 +146,11 @@ int main() {
   std::printf(
       "{\"rhs\": %lld, \"rk4_step\": %lld, \"dp45_attempt\": %lld, \"crossing\": %lld, "
       "\"disk_hit\": %lld, \"volume_sample\": %lld, \"setup\": %lld, \"projection\": %lld, "
+      "\"volume_step_nosample\": %lld, "
       "\"trans\": {\"rhs\": %lld, \"rk4_step\": %lld, \"dp45_attempt\": %lld, \"crossing\": %lld, "
       "\"disk_hit\": %lld, \"volume_sample\": %lld, \"setup\": %lld}}\n",
       t_rhs.flop, t_rk4.flop, t_dp.flop, t_cross.flop, t_disk.flop, t_vol.flop, t_setup.flop,
-      t_proj.flop,
+      t_proj.flop, t_vol_skip.flop,
       t_rhs.trans, t_rk4.trans, t_dp.trans, t_cross.trans, t_disk.trans, t_vol.trans,
       t_setup.trans);
   return 0;

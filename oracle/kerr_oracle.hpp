@@ -455,13 +455,8 @@ struct VolumeHost {
   const float* cells;  // host float4 grid
   BH_Volume v;
 };
-struct K2Host {
-  const double* tab;
-  BH_K2Table t;
-};
-
 template <class T>
-inline void volume_fetch(const VolumeHost& V, T r, T u, T phi, T* out3) {
+inline void volume_fetch(const VolumeHost& V, T r, T u, T phi, T* out4) {
   const BH_Volume& v = V.v;
   const T theta = acos_(u);
   const T xr = (r - T(v.r_min)) / T(v.r_max - v.r_min) * T(double(v.n_r)) - T(0.5);
@@ -481,19 +476,19 @@ inline void volume_fetch(const VolumeHost& V, T r, T u, T phi, T* out3) {
   ip0 = ((ip0 % v.n_ph) + v.n_ph) % v.n_ph;
   const long long ip1 = (ip0 + 1) % v.n_ph;
   const long long ir[2] = {ir0, ir1}, it[2] = {it0, it1}, ip[2] = {ip0, ip1};
-  for (int f = 0; f < 3; ++f) {
-    T c[2][2];
+  for (int f = 0; f < 4; ++f) {
+    T cc[2][2];
     for (int b = 0; b < 2; ++b) {
       for (int a = 0; a < 2; ++a) {
         const long long base = (ip[b] * v.n_th + it[a]) * v.n_r;
         const T v0 = T(double(V.cells[4 * (base + ir[0]) + f]));
         const T v1 = T(double(V.cells[4 * (base + ir[1]) + f]));
-        c[b][a] = fma_(fr, v1 - v0, v0);
+        cc[b][a] = fma_(fr, v1 - v0, v0);
       }
     }
-    const T c0 = fma_(ft, c[0][1] - c[0][0], c[0][0]);
-    const T c1 = fma_(ft, c[1][1] - c[1][0], c[1][0]);
-    out3[f] = fma_(fp, c1 - c0, c0);
+    const T c0 = fma_(ft, cc[0][1] - cc[0][0], cc[0][0]);
+    const T c1 = fma_(ft, cc[1][1] - cc[1][0], cc[1][0]);
+    out4[f] = fma_(fp, c1 - c0, c0);
   }
 }
 
@@ -521,13 +516,6 @@ inline T fluid_g(T r, T u, T a, T xi, T ell) {
 // Thermal synchrotron emissivity of src/physics/rte_integrator.h:291-328
 // (Mahadevan 1996 fit), with K_2(1/Theta) = 2 Theta^2 exp(tab(ln Theta)).
 template <class T>
-inline T k2_ratio(const K2Host& K, T theta_e) {
-  // K.tab holds ln(K_2(1/Theta) / (2 Theta^2)) on a uniform ln(Theta) grid.
-  const T lt = log_(theta_e);
-  return exp_(lut_linear(K.tab, K.t.n, T(K.t.lnth_min), T(K.t.lnth_max), lt));
-}
-
-template <class T>
 inline T synch_j(T nu, T B, T ne, T th, T k2r) {
   // k2r = K_2(1/Theta) / (2 Theta^2)
   const T nuB = T(kECharge) * B / T(kTwoPi * kMElectron * kC);
@@ -550,6 +538,76 @@ inline T planck(T nu, T tempK) {
   const T x = T(kH / kKB) * nu / tempK;
   if (val_(x) > 700.0) return T(0.0);
   return T(2.0 * kH / (kC * kC)) * nu * nu * nu / expm1_(x);
+}
+
+// Emission and absorption per unit affine parameter at a point of the
+// volume, in the observer's frame: with the fluid redshift g, nu_f = nu/g,
+// alpha_nu = j_nu / B_nu(T_e) (Kirchhoff, rte_integrator.h:348-357) and the
+// invariants I/nu^3, j/nu^2, nu alpha:
+//   a = alpha_nu(nu_f) L / g      (d tau      = a d lambda_affine)
+//   e = g^2 j_nu(nu_f) L          (d I_obs    = e d lambda_affine)
+// L = GM/c^2 in cm.  Returns false when the point lies outside the grid or
+// the support (no sample taken); a = e = 0 where the gas is empty.
+template <class T>
+inline bool volume_point(const VolumeHost& V, const RayConst<T>& c, T r, T u, T phi,
+                         double nu_obs, double L, T* a, T* e) {
+  const BH_Volume& v = V.v;
+  *a = T(0.0);
+  *e = T(0.0);
+  if (!(val_(r) >= v.r_min && val_(r) <= v.r_max && val_(r) >= v.sup_r_lo &&
+        val_(r) <= v.sup_r_hi && std::fabs(val_(u)) <= v.sup_u_max))
+    return false;
+  T f4[4];
+  volume_fetch(V, r, u, phi, f4);
+  const T ne = f4[0], th = f4[1], B = f4[2], k2r = f4[3];
+  if (!(val_(ne) > 0.0 && val_(th) > 0.0 && val_(B) > 0.0 && val_(k2r) > 0.0)) return true;
+  const T g = fluid_g(r, u, c.a, c.xi, T(v.ell));
+  if (!(val_(g) > 0.0)) return true;
+  const T nu_f = T(nu_obs) / g;
+  const T tempK = th * T(kMElectron * kC * kC / kKB);
+  const T Bnu = planck(nu_f, tempK);
+  if (!(val_(Bnu) > 0.0)) return true;
+  const T jL = synch_j(nu_f, B, ne, th, k2r) * T(L);
+  *a = jL / (g * Bnu);
+  *e = g * g * jL;
+  return true;
+}
+
+// d lambda_affine / ds = Sigma / r.
+template <class T>
+inline T affine_rate(const RayConst<T>& c, T r, T u) {
+  return fma_(c.a2, u * u, r * r) / r;
+}
+
+// Front-to-back formal solution over one step of length h between two
+// samples (a0, e0, s0) -> (a1, e1, s1), trapezoidal in the affine parameter:
+//   d lambda = h (s0 + s1)/2,  d tau = (a0 + a1) d lambda / 2,
+//   S = (e0 + e1)/(a0 + a1),   I += T S (1 - e^-dtau),  T *= e^-dtau.
+// Second-order accurate (the reference's d_rte_step, device_physics.cuh:
+// 1988-2015, is the first-order right-endpoint form of the same update).
+template <class T>
+struct RtAcc {
+  T I, Tr, tau, a0, e0, s0;
+};
+template <class T>
+inline void rt_segment(RtAcc<T>& q, T h, T a1, T e1, T s1) {
+  const T asum = q.a0 + a1;
+  const T esum = q.e0 + e1;
+  if (val_(asum) > 0.0 || val_(esum) > 0.0) {
+    const T dl = T(0.5) * h * (q.s0 + s1);
+    if (val_(asum) > 0.0) {
+      const T dtau = T(0.5) * asum * dl;
+      const T om = -expm1_(-dtau);
+      q.I = fma_(q.Tr * (esum / asum), om, q.I);
+      q.Tr = q.Tr * (T(1.0) - om);
+      q.tau = q.tau + dtau;
+    } else {
+      q.I = fma_(q.Tr, T(0.5) * esum * dl, q.I);
+    }
+  }
+  q.a0 = a1;
+  q.e0 = e1;
+  q.s0 = s1;
 }
 
 }  // namespace orc

@@ -28,48 +28,27 @@ struct Scene {
   int lut_n;
   bool vol_on;
   VolumeHost V;
-  K2Host K;
 };
 
 inline bool in_support(const BH_Volume& v, double r, double u) {
   return r >= v.sup_r_lo && r <= v.sup_r_hi && std::fabs(u) <= v.sup_u_max;
 }
 
-// Radiative work after an accepted step of length h ending at y (volume
-// segment, then equatorial crossing / disk).  Returns true when the ray ends.
+// Radiative state of one ray.
 struct RayAcc {
-  double I = 0.0, Tr = 1.0, tau = 0.0;
+  RtAcc<double> rt{0.0, 1.0, 0.0, 0.0, 0.0, 0.0};
   int ncross = 0;
   int nsamples = 0;
   double cross_r[3] = {0, 0, 0}, cross_g[3] = {0, 0, 0};
 };
 
+// Volume transfer over the accepted step of length h that ended at y.
 inline void volume_segment(const Scene& S, const RayConst<double>& c, const double* y, double h,
                            RayAcc& acc) {
-  const BH_Volume& v = S.V.v;
-  const double r = y[0], u = y[1];
-  if (!(r >= v.r_min && r <= v.r_max && in_support(v, r, u))) return;
-  double f3[3];
-  volume_fetch(S.V, r, u, y[2], f3);
-  acc.nsamples++;
-  const double ne = f3[0], th = f3[1], B = f3[2];
-  if (!(ne > 0.0 && th > 0.0 && B > 0.0)) return;
-  const double g = fluid_g(r, u, c.a, c.xi, v.ell);
-  if (!(g > 0.0)) return;
-  const double nu_f = S.o->nu_obs / g;
-  const double tempK = th * (kMElectron * kC * kC / kKB);
-  const double Bnu = planck(nu_f, tempK);
-  if (!(Bnu > 0.0)) return;
-  const double j = synch_j(nu_f, B, ne, th, k2_ratio(S.K, th));
-  const double alpha_nu = j / Bnu;
-  const double sig = std::fma(c.a2, u * u, r * r);
-  const double dl = h * sig / r;  // affine length of the segment (Mino ds/r times Sigma)
-  const double dtau = alpha_nu * (dl / g) * S.o->length_unit_cm;
-  const double om = -std::expm1(-dtau);
-  const double g3 = g * g * g;
-  acc.I = std::fma(acc.Tr * g3 * Bnu, om, acc.I);
-  acc.Tr = acc.Tr * (1.0 - om);
-  acc.tau = acc.tau + dtau;
+  double a1, e1;
+  if (volume_point(S.V, c, y[0], y[1], y[2], S.o->nu_obs, S.o->length_unit_cm, &a1, &e1))
+    acc.nsamples++;
+  rt_segment(acc.rt, h, a1, e1, affine_rate(c, y[0], y[1]));
 }
 
 // Returns true when the disk terminated the ray.
@@ -86,7 +65,7 @@ inline bool crossing(const Scene& S, const CamConst& k, const RayConst<double>& 
   if (acc.ncross < 255) acc.ncross++;
   if (S.o->disk_enabled && rc >= k.r_isco && rc <= S.o->disk_r_out) {
     const double F = lut_linear(S.luts, S.lut_n, k.r_isco, S.o->disk_r_out, rc);
-    acc.I = std::fma(acc.Tr, disk_intensity(F, g, S.o->nu_obs, S.o->disk_T_max), acc.I);
+    acc.rt.I = std::fma(acc.rt.Tr, disk_intensity(F, g, S.o->nu_obs, S.o->disk_T_max), acc.rt.I);
     return true;
   }
   return false;
@@ -95,7 +74,7 @@ inline bool crossing(const Scene& S, const CamConst& k, const RayConst<double>& 
 inline void finish(const RayConst<double>& c, const double* y, const RayAcc& acc, int status,
                    int steps, int attempts, BH_PixelRecord* rec) {
   std::memset(rec, 0, sizeof(*rec));
-  rec->intensity = acc.I;
+  rec->intensity = acc.rt.I;
   rec->r_exit = y[0];
   rec->u_exit = y[1];
   rec->phi_exit = y[2];
@@ -103,7 +82,7 @@ inline void finish(const RayConst<double>& c, const double* y, const RayAcc& acc
     rec->cross_r[i] = acc.cross_r[i];
     rec->cross_g[i] = acc.cross_g[i];
   }
-  rec->tau = acc.tau;
+  rec->tau = acc.rt.tau;
   const double R = radial_potential(c, y[0]);
   const double U = polar_potential(c, y[1]);
   const double rs = y[0] * y[0] + c.a2;
@@ -122,7 +101,7 @@ inline void finish(const RayConst<double>& c, const double* y, const RayAcc& acc
 inline int terminate_check(const Scene& S, const CamConst& k, const double* y, const RayAcc& acc) {
   if (y[0] < k.r_stop) return BH_HIT_HORIZON;
   if (y[0] > S.o->r_escape) return BH_HIT_ESCAPED;
-  if (S.o->transmit_min > 0.0 && acc.Tr < S.o->transmit_min) return BH_HIT_OPAQUE;
+  if (S.o->transmit_min > 0.0 && acc.rt.Tr < S.o->transmit_min) return BH_HIT_OPAQUE;
   return 0;
 }
 
@@ -131,6 +110,7 @@ void trace_rk4(const Scene& S, const CamConst& k, int64_t px, int64_t py, BH_Pix
   const RayConst<double> c = setup_ray<double>(k, px, py, y);
   const double h = S.o->h_fixed;
   RayAcc acc;
+  if (S.vol_on) acc.rt.s0 = affine_rate(c, y[0], y[1]);
   int steps = 0, status = 0;
   rhs(c, y, k1);
   for (;;) {
@@ -161,6 +141,7 @@ void trace_dp45(const Scene& S, const CamConst& k, int64_t px, int64_t py, BH_Pi
   const RayConst<double> c = setup_ray<double>(k, px, py, y);
   const BH_TraceOptions& o = *S.o;
   RayAcc acc;
+  if (S.vol_on) acc.rt.s0 = affine_rate(c, y[0], y[1]);
   int steps = 0, attempts = 0, status = 0;
   bool last_rej = false;
   double h = o.h_init;
@@ -215,22 +196,20 @@ extern "C" {
 
 // Trace the rays with global record indices ids[0..n_ids) (or all rays when
 // ids is NULL) of the camera set; out[i] receives the record of ids[i].
-// volume_cells / k2_tab are host arrays; nthreads <= 0 uses all cores.
+// volume_cells is a host array; nthreads <= 0 uses all cores.
 int orc_trace(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
               const double* luts, int lut_n, const BH_Volume* vol, const float* volume_cells,
-              const BH_K2Table* k2, const double* k2_tab, const int64_t* ids, int64_t n_ids,
-              BH_PixelRecord* out, int nthreads) {
+              const int64_t* ids, int64_t n_ids, BH_PixelRecord* out, int nthreads) {
   if (!cams || n_cams <= 0 || !opts || !out) return -1;
   Scene S;
   S.o = opts;
   S.luts = luts;
   S.lut_n = lut_n;
-  S.vol_on = opts->volume_enabled && vol && volume_cells && k2 && k2_tab;
+  S.vol_on = opts->volume_enabled && vol && volume_cells;
+  if (opts->volume_enabled && !S.vol_on) return -1;
   if (S.vol_on) {
     S.V.cells = volume_cells;
     S.V.v = *vol;
-    S.K.tab = k2_tab;
-    S.K.t = *k2;
   }
   if (opts->disk_enabled && (!luts || lut_n < 2)) return -1;
   std::vector<CamConst> ks;
@@ -346,11 +325,32 @@ double orc_lut_linear(const double* tab, int n, double x0, double x1, double x) 
   return lut_linear(tab, n, x0, x1, x);
 }
 void orc_volume_fetch(const BH_Volume* v, const float* cells, double r, double u, double phi,
-                      double* out3) {
+                      double* out4) {
   VolumeHost V;
   V.cells = cells;
   V.v = *v;
-  volume_fetch(V, r, u, phi, out3);
+  volume_fetch(V, r, u, phi, out4);
+}
+// Observer-frame absorption / emission per affine length at a point
+// (volume_point); returns 1 if a sample was taken.
+int orc_volume_point(const BH_Volume* v, const float* cells, double a, double xi, double eta,
+                     double r, double u, double phi, double nu_obs, double L, double* ae) {
+  VolumeHost V;
+  V.cells = cells;
+  V.v = *v;
+  const RayConst<double> c = make_const(a, xi, eta);
+  return volume_point(V, c, r, u, phi, nu_obs, L, &ae[0], &ae[1]) ? 1 : 0;
+}
+// One trapezoidal transfer segment (rt_segment): st = {I, T, tau, a0, e0, s0}.
+void orc_rt_segment(double* st, double h, double a1, double e1, double s1) {
+  RtAcc<double> q{st[0], st[1], st[2], st[3], st[4], st[5]};
+  rt_segment(q, h, a1, e1, s1);
+  st[0] = q.I;
+  st[1] = q.Tr;
+  st[2] = q.tau;
+  st[3] = q.a0;
+  st[4] = q.e0;
+  st[5] = q.s0;
 }
 
 // Normalised Novikov-Thorne flux table (the reference's NT approximation,

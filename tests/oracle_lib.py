@@ -35,8 +35,7 @@ def lib():
         L.orc_trace.argtypes = [
             C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions),
             _PD, C.c_int, C.POINTER(abi.BH_Volume), C.POINTER(C.c_float),
-            C.POINTER(abi.BH_K2Table), _PD, C.POINTER(C.c_int64), C.c_int64,
-            C.c_void_p, C.c_int,
+            C.POINTER(C.c_int64), C.c_int64, C.c_void_p, C.c_int,
         ]
         L.orc_trace.restype = C.c_int
         L.orc_rhs.argtypes = [_D, _D, _D, _PD, _PD]
@@ -71,6 +70,9 @@ def lib():
         L.orc_lut_linear.argtypes = [_PD, C.c_int, _D, _D, _D]
         L.orc_lut_linear.restype = _D
         L.orc_volume_fetch.argtypes = [C.POINTER(abi.BH_Volume), C.POINTER(C.c_float), _D, _D, _D, _PD]
+        L.orc_volume_point.argtypes = [C.POINTER(abi.BH_Volume), C.POINTER(C.c_float)] + [_D] * 8 + [_PD]
+        L.orc_volume_point.restype = C.c_int
+        L.orc_rt_segment.argtypes = [_PD, _D, _D, _D, _D]
         L.orc_disk_lut.argtypes = [_D, _D, _D, C.c_int, _PD]
         L.orc_bessel_k.argtypes = [_D, _D]
         L.orc_bessel_k.restype = _D
@@ -97,8 +99,8 @@ def k2_table(n: int, lo: float, hi: float) -> np.ndarray:
     return out
 
 
-def trace(cams, opts, luts=None, lut_n=0, vol=None, cells=None, k2=None, k2_tab=None,
-          ids=None, nthreads=0) -> np.ndarray:
+def trace(cams, opts, luts=None, lut_n=0, vol=None, cells=None, ids=None,
+          nthreads=0) -> np.ndarray:
     """Trace with the oracle; returns a RECORD_DTYPE array (one per id, or all rays)."""
     cam_arr = (abi.BH_Camera * len(cams))(*cams)
     total = max(c.record_offset + c.width * c.height for c in cams)
@@ -112,8 +114,6 @@ def trace(cams, opts, luts=None, lut_n=0, vol=None, cells=None, k2=None, k2_tab=
         dptr(luts) if luts is not None else None, lut_n,
         C.byref(vol) if vol is not None else None,
         cells.ctypes.data_as(C.POINTER(C.c_float)) if cells is not None else None,
-        C.byref(k2) if k2 is not None else None,
-        dptr(k2_tab) if k2_tab is not None else None,
         ids_arr.ctypes.data_as(C.POINTER(C.c_int64)) if ids_arr is not None else None,
         n if ids is not None else 0,
         out.ctypes.data_as(C.c_void_p), nthreads,

@@ -8,15 +8,11 @@ import numpy as np
 from blackhole_amd import abi, tracer
 
 
-def k2_struct(k2tab: np.ndarray) -> abi.BH_K2Table:
-    return abi.BH_K2Table(None, k2tab.size, 0, tracer.K2_LNTH[0], tracer.K2_LNTH[1])
-
-
 def run_gpu(w: dict) -> np.ndarray:
     """Trace workload dict w on cuda:0 through the C ABI; returns records (numpy)."""
     import torch
     sc = tracer.upload_scene(w["cams"], w["opts"], luts=w.get("luts"), cells=w.get("cells"),
-                             vol=w.get("vol"), k2tab=w.get("k2tab"))
+                             vol=w.get("vol"))
     out = tracer.alloc_records(sc.n_records)
     out.fill_(0)
     tracer.launch(sc, out)
@@ -28,9 +24,7 @@ def run_oracle(oracle, w: dict, ids=None) -> np.ndarray:
     cells = w.get("cells")
     return oracle.trace(
         w["cams"], w["opts"], luts=w.get("luts"), lut_n=(w["luts"].shape[-1] if w.get("luts") is not None else 0),
-        vol=w.get("vol"), cells=cells.reshape(-1) if cells is not None else None,
-        k2=k2_struct(w["k2tab"]) if w.get("k2tab") is not None else None, k2_tab=w.get("k2tab"),
-        ids=ids)
+        vol=w.get("vol"), cells=cells.reshape(-1) if cells is not None else None, ids=ids)
 
 
 def rel(a, b):

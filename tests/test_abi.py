@@ -59,10 +59,17 @@ def test_disk_lut_matches_oracle_bitwise(oracle):
         assert got.tobytes() == ref.tobytes()
 
 
-def test_k2_table_matches_oracle(oracle):
-    got = tracer.k2_table(64)
-    ref = oracle.k2_table(64, tracer.K2_LNTH[0], tracer.K2_LNTH[1])
-    np.testing.assert_array_equal(got, ref)
+def test_torus_k2_lane_matches_oracle_bessel(oracle):
+    """The generator's fourth cell lane is K_2(1/Theta_e)/(2 Theta_e^2) of the
+    cell's own Theta_e (the oracle's quadrature; pinned against scipy kv)."""
+    p = tracer.torus_params(0.94, n_r=32, n_th=16, n_ph=8)
+    cells, vol = tracer.generate_torus(p)
+    flat = cells.reshape(-1, 4)
+    live = flat[flat[:, 1] > 0]
+    assert live.shape[0] > 10
+    for th, k2r in live[:: max(1, live.shape[0] // 40), 1:4:2]:
+        ref = oracle.lib().orc_bessel_k(2.0, 1.0 / float(th)) / (2.0 * float(th) ** 2)
+        assert abs(float(k2r) - ref) <= 1e-6 * ref
 
 
 @pytest.mark.parametrize("field,value", [
@@ -71,7 +78,7 @@ def test_invalid_options_rejected_without_gpu(field, value):
     o = tracer.default_options()
     setattr(o, field, value)
     cams = (abi.BH_Camera * 1)(tracer.camera(0.9, 60.0, 100.0, 8, 8))
-    rc = lib().bh_trace_image_plane(cams, 1, C.byref(o), C.c_void_p(1), 256, None, None,
+    rc = lib().bh_trace_image_plane(cams, 1, C.byref(o), C.c_void_p(1), 256, None,
                                     C.c_void_p(1), None)
     assert rc == -1
     assert lib().bh_last_error()
@@ -81,19 +88,19 @@ def test_invalid_options_rejected_without_gpu(field, value):
 def test_invalid_camera_rejected_without_gpu(spin, theta, w):
     o = tracer.default_options()
     cams = (abi.BH_Camera * 1)(abi.BH_Camera(spin, theta, 100.0, -1, 1, -1, 1, w, 8, 0))
-    rc = lib().bh_trace_image_plane(cams, 1, C.byref(o), C.c_void_p(1), 256, None, None,
+    rc = lib().bh_trace_image_plane(cams, 1, C.byref(o), C.c_void_p(1), 256, None,
                                     C.c_void_p(1), None)
     assert rc == -1
 
 
 def test_null_arguments_rejected():
     o = tracer.default_options()
-    assert lib().bh_trace_image_plane(None, 0, C.byref(o), None, 0, None, None, None, None) == -1
+    assert lib().bh_trace_image_plane(None, 0, C.byref(o), None, 0, None, None, None) == -1
     cams = (abi.BH_Camera * 1)(tracer.camera(0.9, 60.0, 100.0, 8, 8))
-    assert lib().bh_trace_image_plane(cams, 1, None, None, 0, None, None, C.c_void_p(1), None) == -1
+    assert lib().bh_trace_image_plane(cams, 1, None, None, 0, None, C.c_void_p(1), None) == -1
     # disk enabled but no LUT
-    assert lib().bh_trace_image_plane(cams, 1, C.byref(o), None, 0, None, None, C.c_void_p(1), None) == -1
+    assert lib().bh_trace_image_plane(cams, 1, C.byref(o), None, 0, None, C.c_void_p(1), None) == -1
     # volume enabled but no volume
     o.volume_enabled = 1
-    assert lib().bh_trace_image_plane(cams, 1, C.byref(o), C.c_void_p(1), 256, None, None,
+    assert lib().bh_trace_image_plane(cams, 1, C.byref(o), C.c_void_p(1), 256, None,
                                       C.c_void_p(1), None) == -1

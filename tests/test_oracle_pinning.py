@@ -489,8 +489,9 @@ def test_bessel_k_quadrature(L):
 
 
 def test_volume_fetch_trilinear_bruteforce(L):
-    """Cell-centred trilinear sampling with r clamp, theta clamp and phi wrap,
-    checked against a direct numpy evaluation on a random 6 x 5 x 4 grid."""
+    """Cell-centred trilinear sampling (four cell fields) with r clamp, theta
+    clamp and phi wrap, checked against a direct numpy evaluation on a random
+    6 x 5 x 4 grid."""
     rng = np.random.default_rng(4)
     nr, nt, npz = 6, 5, 4
     cells = rng.uniform(0, 1, (npz, nt, nr, 4)).astype(np.float32)
@@ -499,8 +500,8 @@ def test_volume_fetch_trilinear_bruteforce(L):
     def brute(r, u, phi):
         th = math.acos(u)
         xs = [(r - 2.0) / 12.0 * nr - 0.5, th / math.pi * nt - 0.5, phi / (2 * math.pi) * npz - 0.5]
-        out = np.zeros(3)
-        for f in range(3):
+        out = np.zeros(4)
+        for f in range(4):
             acc = 0.0
             i0 = [math.floor(x) for x in xs]
             fr = [x - math.floor(x) for x in xs]
@@ -518,7 +519,7 @@ def test_volume_fetch_trilinear_bruteforce(L):
     flat = np.ascontiguousarray(cells.reshape(-1))
     for _ in range(300):
         r, u, phi = rng.uniform(1.0, 15.0), rng.uniform(-1, 1), rng.uniform(-20, 20)
-        got = np.zeros(3)
+        got = np.zeros(4)
         L.orc_volume_fetch(C.byref(vol), flat.ctypes.data_as(C.POINTER(C.c_float)), r, u, phi, P(got))
         np.testing.assert_allclose(got, brute(r, u, phi), rtol=1e-12, atol=1e-13)
 
@@ -535,6 +536,7 @@ def test_flop_constants_match_counting_oracle():
     assert got["crossing"] == flops.CROSSING
     assert got["disk_hit"] == flops.DISK_HIT
     assert got["volume_sample"] == flops.VOLUME_SAMPLE
+    assert got["volume_step_nosample"] == flops.VOLUME_STEP_NOSAMPLE
     assert got["setup"] == flops.SETUP
 
 
