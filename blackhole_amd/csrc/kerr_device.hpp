@@ -35,14 +35,43 @@ constexpr double kTwoPi = 2.0 * kPi;
 constexpr double kH = 2.0 * kPi * kHbar;
 constexpr double kProjKappa = 0.0625;
 
-struct RayC {
-  double a, a2, xi, eta, K, c1, c2, two_a2;
+// Scalar helpers: the FP64 instantiation uses correctly rounded operations
+// (parity with the oracle); the FP32 instantiation uses the fast hardware
+// forms (v_rcp_f32 / v_sqrt_f32 based), which the precision form's
+// tolerance admits.
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double divT(double a, double b) { return a / b; }
+__device__ __forceinline__ float divT(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ double sqrtT(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float sqrtT(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double fabsT(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float fabsT(float x) { return __builtin_fabsf(x); }
+
+template <class T>
+struct RayCT {
+  T a, a2, xi, eta, K, c1, c2, two_a2;
 };
+using RayC = RayCT<double>;
+
+template <class T>
+__device__ __forceinline__ RayCT<T> convert_const(const RayC& c) {
+  RayCT<T> o;
+  o.a = static_cast<T>(c.a);
+  o.a2 = static_cast<T>(c.a2);
+  o.xi = static_cast<T>(c.xi);
+  o.eta = static_cast<T>(c.eta);
+  o.K = static_cast<T>(c.K);
+  o.c1 = static_cast<T>(c.c1);
+  o.c2 = static_cast<T>(c.c2);
+  o.two_a2 = static_cast<T>(c.two_a2);
+  return o;
+}
 
 // Per-camera constants, computed on the host (one libm sin/cos/cbrt per
 // camera) and read by every lane that traces a pixel of the camera.
 struct CamDev {
-  double a, sin_o, cos_o, r_obs, r_stop, r_isco;
+  double a, sin_o, cos_o, r_obs, r_stop, r_isco, r_plus;
   double alpha_min, dalpha, beta_max, dbeta;
   int64_t w, h, offset, lut_index;
 };
@@ -61,43 +90,47 @@ __device__ __forceinline__ RayC make_const(double a, double xi, double eta) {
   return c;
 }
 
-__device__ __forceinline__ void rhs(const RayC& c, const double* y, double* f) {
-  const double r = y[0];
-  const double u = y[1];
-  const double r2 = r * r;
-  const double P = r2 + c.c1;
-  const double Delta = __builtin_fma(r - 2.0, r, c.a2);
-  double w = __builtin_fma(-u, u, 1.0);
-  if (w < 1e-14) w = 1e-14;
-  const double q = 1.0 / (r * Delta * w);
-  const double inv_r = Delta * w * q;
+template <class T>
+__device__ __forceinline__ void rhs(const RayCT<T>& c, const T* y, T* f) {
+  const T r = y[0];
+  const T u = y[1];
+  const T r2 = r * r;
+  const T P = r2 + c.c1;
+  const T Delta = fmaT(r - T(2.0), r, c.a2);
+  T w = fmaT(-u, u, T(1.0));
+  if (w < T(1e-14)) w = T(1e-14);
+  const T q = divT(T(1.0), r * Delta * w);
+  const T inv_r = Delta * w * q;
   f[0] = y[3] * inv_r;
   f[1] = y[4] * inv_r;
-  f[2] = __builtin_fma(c.xi, Delta, c.a * w * (P - Delta)) * q;
-  f[3] = __builtin_fma(2.0 * r, P, -((r - 1.0) * c.K)) * inv_r;
-  f[4] = u * __builtin_fma(-c.two_a2, u * u, c.c2) * inv_r;
+  f[2] = fmaT(c.xi, Delta, c.a * w * (P - Delta)) * q;
+  f[3] = fmaT(T(2.0) * r, P, -((r - T(1.0)) * c.K)) * inv_r;
+  f[4] = u * fmaT(-c.two_a2, u * u, c.c2) * inv_r;
 }
 
-__device__ __forceinline__ double radial_potential(const RayC& c, double r) {
-  const double P = r * r + c.c1;
-  const double Delta = __builtin_fma(r - 2.0, r, c.a2);
-  return __builtin_fma(P, P, -(Delta * c.K));
+template <class T>
+__device__ __forceinline__ T radial_potential(const RayCT<T>& c, T r) {
+  const T P = r * r + c.c1;
+  const T Delta = fmaT(r - T(2.0), r, c.a2);
+  return fmaT(P, P, -(Delta * c.K));
 }
-__device__ __forceinline__ double polar_potential(const RayC& c, double u) {
-  const double u2 = u * u;
-  return __builtin_fma(__builtin_fma(-c.a2, u2, c.c2), u2, c.eta);
+template <class T>
+__device__ __forceinline__ T polar_potential(const RayCT<T>& c, T u) {
+  const T u2 = u * u;
+  return fmaT(fmaT(-c.a2, u2, c.c2), u2, c.eta);
 }
 
 // Radial null-constraint projection where R is far from zero (the
 // reference's null renormalisation, src/physics/verified/null_constraint.hpp:
 // 208-271, restricted to the radial first integral).
-__device__ __forceinline__ void project_radial(const RayC& c, double* y, double* f) {
-  const double R = radial_potential(c, y[0]);
-  const double s = __builtin_fma(y[0], y[0], c.a2);
-  if (R > kProjKappa * (s * s)) {
-    const double vn = __builtin_sqrt(R);
-    const double v = (y[3] < 0.0) ? -vn : vn;
-    f[0] = f[0] * (v / y[3]);
+template <class T>
+__device__ __forceinline__ void project_radial(const RayCT<T>& c, T* y, T* f) {
+  const T R = radial_potential(c, y[0]);
+  const T s = fmaT(y[0], y[0], c.a2);
+  if (R > T(kProjKappa) * (s * s)) {
+    const T vn = sqrtT(R);
+    const T v = (y[3] < T(0.0)) ? -vn : vn;
+    f[0] = f[0] * divT(v, y[3]);
     y[3] = v;
   }
 }
@@ -121,24 +154,24 @@ __device__ __forceinline__ RayC setup_ray(const CamDev& k, int64_t px, int64_t p
 }
 
 // Classic RK4 step from y with k1 = f(y).
-__device__ __forceinline__ void rk4_step(const RayC& c, const double* y, const double* k1, double h,
-                                         double* yn) {
-  double k2[kN], k3[kN], k4[kN], yt[kN];
-  const double hh = 0.5 * h;
-  const double h6 = h / 6.0;
+template <class T>
+__device__ __forceinline__ void rk4_step(const RayCT<T>& c, const T* y, const T* k1, T h, T* yn) {
+  T k2[kN], k3[kN], k4[kN], yt[kN];
+  const T hh = T(0.5) * h;
+  const T h6 = divT(h, T(6.0));
 #pragma unroll
-  for (int i = 0; i < kN; ++i) yt[i] = __builtin_fma(hh, k1[i], y[i]);
+  for (int i = 0; i < kN; ++i) yt[i] = fmaT(hh, k1[i], y[i]);
   rhs(c, yt, k2);
 #pragma unroll
-  for (int i = 0; i < kN; ++i) yt[i] = __builtin_fma(hh, k2[i], y[i]);
+  for (int i = 0; i < kN; ++i) yt[i] = fmaT(hh, k2[i], y[i]);
   rhs(c, yt, k3);
 #pragma unroll
-  for (int i = 0; i < kN; ++i) yt[i] = __builtin_fma(h, k3[i], y[i]);
+  for (int i = 0; i < kN; ++i) yt[i] = fmaT(h, k3[i], y[i]);
   rhs(c, yt, k4);
 #pragma unroll
   for (int i = 0; i < kN; ++i) {
-    const double s = __builtin_fma(2.0, k2[i] + k3[i], k1[i] + k4[i]);
-    yn[i] = __builtin_fma(h6, s, y[i]);
+    const T s = fmaT(T(2.0), k2[i] + k3[i], k1[i] + k4[i]);
+    yn[i] = fmaT(h6, s, y[i]);
   }
 }
 
@@ -159,61 +192,53 @@ constexpr double e1 = 71.0 / 57600.0, e3 = -71.0 / 16695.0, e4 = 71.0 / 1920.0,
                  e5 = -17253.0 / 339200.0, e6 = 22.0 / 525.0, e7 = -1.0 / 40.0;
 }  // namespace dp
 
-__device__ __forceinline__ double dp45_attempt(const RayC& c, const double* y, const double* k1,
-                                               double h, double rtol, double atol, double* y5,
-                                               double* k7) {
+template <class T>
+__device__ __forceinline__ T dp45_attempt(const RayCT<T>& c, const T* y, const T* k1, T h, T rtol,
+                                          T atol, T* y5, T* k7) {
   using namespace dp;
-  double k2[kN], k3[kN], k4[kN], k5[kN], k6[kN], yt[kN];
+  T k2[kN], k3[kN], k4[kN], k5[kN], k6[kN], yt[kN];
 #pragma unroll
-  for (int i = 0; i < kN; ++i) yt[i] = __builtin_fma(h, a21 * k1[i], y[i]);
+  for (int i = 0; i < kN; ++i) yt[i] = fmaT(h, T(a21) * k1[i], y[i]);
   rhs(c, yt, k2);
 #pragma unroll
-  for (int i = 0; i < kN; ++i) yt[i] = __builtin_fma(h, __builtin_fma(a32, k2[i], a31 * k1[i]), y[i]);
+  for (int i = 0; i < kN; ++i) yt[i] = fmaT(h, fmaT(T(a32), k2[i], T(a31) * k1[i]), y[i]);
   rhs(c, yt, k3);
 #pragma unroll
   for (int i = 0; i < kN; ++i)
-    yt[i] = __builtin_fma(h, __builtin_fma(a43, k3[i], __builtin_fma(a42, k2[i], a41 * k1[i])), y[i]);
+    yt[i] = fmaT(h, fmaT(T(a43), k3[i], fmaT(T(a42), k2[i], T(a41) * k1[i])), y[i]);
   rhs(c, yt, k4);
 #pragma unroll
   for (int i = 0; i < kN; ++i)
-    yt[i] = __builtin_fma(
-        h,
-        __builtin_fma(a54, k4[i], __builtin_fma(a53, k3[i], __builtin_fma(a52, k2[i], a51 * k1[i]))),
-        y[i]);
+    yt[i] = fmaT(h, fmaT(T(a54), k4[i], fmaT(T(a53), k3[i], fmaT(T(a52), k2[i], T(a51) * k1[i]))),
+                 y[i]);
   rhs(c, yt, k5);
 #pragma unroll
   for (int i = 0; i < kN; ++i)
-    yt[i] = __builtin_fma(
-        h,
-        __builtin_fma(a65, k5[i],
-                      __builtin_fma(a64, k4[i],
-                                    __builtin_fma(a63, k3[i], __builtin_fma(a62, k2[i], a61 * k1[i])))),
-        y[i]);
+    yt[i] = fmaT(h,
+                 fmaT(T(a65), k5[i],
+                      fmaT(T(a64), k4[i], fmaT(T(a63), k3[i], fmaT(T(a62), k2[i], T(a61) * k1[i])))),
+                 y[i]);
   rhs(c, yt, k6);
 #pragma unroll
   for (int i = 0; i < kN; ++i)
-    y5[i] = __builtin_fma(
-        h,
-        __builtin_fma(b6, k6[i],
-                      __builtin_fma(b5, k5[i], __builtin_fma(b4, k4[i], __builtin_fma(b3, k3[i], b1 * k1[i])))),
-        y[i]);
+    y5[i] = fmaT(h,
+                 fmaT(T(b6), k6[i],
+                      fmaT(T(b5), k5[i], fmaT(T(b4), k4[i], fmaT(T(b3), k3[i], T(b1) * k1[i])))),
+                 y[i]);
   rhs(c, y5, k7);
-  double acc = 0.0;
+  T acc = T(0.0);
 #pragma unroll
   for (int i = 0; i < kN; ++i) {
-    const double e =
-        h * __builtin_fma(e7, k7[i],
-                          __builtin_fma(e6, k6[i],
-                                        __builtin_fma(e5, k5[i],
-                                                      __builtin_fma(e4, k4[i],
-                                                                    __builtin_fma(e3, k3[i], e1 * k1[i])))));
-    const double ay = __builtin_fabs(y[i]);
-    const double ay5 = __builtin_fabs(y5[i]);
-    const double sc = __builtin_fma(rtol, (ay > ay5) ? ay : ay5, atol);
-    const double qq = e / sc;
-    acc = __builtin_fma(qq, qq, acc);
+    const T e = h * fmaT(T(e7), k7[i],
+                         fmaT(T(e6), k6[i],
+                              fmaT(T(e5), k5[i], fmaT(T(e4), k4[i], fmaT(T(e3), k3[i], T(e1) * k1[i])))));
+    const T ay = fabsT(y[i]);
+    const T ay5 = fabsT(y5[i]);
+    const T sc = fmaT(rtol, (ay > ay5) ? ay : ay5, atol);
+    const T qq = divT(e, sc);
+    acc = fmaT(qq, qq, acc);
   }
-  return __builtin_sqrt(acc * 0.2);
+  return sqrtT(acc * T(0.2));
 }
 
 // x^(-1/5) from exact exponent split + two Newton steps (bit-reproducible).
@@ -270,18 +295,21 @@ __device__ __forceinline__ double hermite_d(double p0, double m0, double p1, dou
   const double d11 = __builtin_fma(3.0, t2, -2.0 * t);
   return __builtin_fma(d00, p0 - p1, __builtin_fma(d10 * h, m0, (d11 * h) * m1));
 }
-__device__ __forceinline__ double crossing_radius(const double* y0, const double* f0,
-                                                  const double* y1, const double* f1, double h) {
-  double t = y0[1] / (y0[1] - y1[1]);
+template <class T>
+__device__ __forceinline__ double crossing_radius(const T* y0, const T* f0, const T* y1,
+                                                  const T* f1, T h) {
+  double t = static_cast<double>(y0[1]) / (static_cast<double>(y0[1]) - static_cast<double>(y1[1]));
+  const double p0 = y0[1], m0 = f0[1], p1 = y1[1], m1 = f1[1], hd = h;
 #pragma unroll
   for (int it = 0; it < 3; ++it) {
-    const double p = hermite(y0[1], f0[1], y1[1], f1[1], h, t);
-    const double dp = hermite_d(y0[1], f0[1], y1[1], f1[1], h, t);
+    const double p = hermite(p0, m0, p1, m1, hd, t);
+    const double dp = hermite_d(p0, m0, p1, m1, hd, t);
     if (dp != 0.0) t = t - p / dp;
     if (t < 0.0) t = 0.0;
     if (t > 1.0) t = 1.0;
   }
-  return hermite(y0[0], f0[0], y1[0], f1[0], h, t);
+  return hermite(static_cast<double>(y0[0]), static_cast<double>(f0[0]), static_cast<double>(y1[0]),
+                 static_cast<double>(f1[0]), hd, t);
 }
 
 // g of a prograde Keplerian emitter at radius r for a photon with L = xi

@@ -30,27 +30,9 @@
 #include <string>
 #include <vector>
 
-#include "kerr_device.hpp"
+#include "trace_common.hpp"
 
 namespace bhk {
-
-constexpr int kBlock = 256;
-
-// Launch parameters, resident in device memory next to the camera table
-// (one upload per launch).  The kernel reads them through a pointer rather
-// than as ~350 bytes of kernel arguments, which kept ~100 scalar registers
-// live across the step loop and spilled them.
-struct DevParams {
-  double h_fixed, h_init, h_max, h_vol, rtol, atol;
-  double r_escape, disk_r_out, nu_obs, disk_T_max, length_unit_cm, transmit_min;
-  VolDev vol;
-  const CamDev* cams;
-  const double* luts;
-  BH_PixelRecord* out;
-  unsigned long long* head;  // ray-queue head
-  long long n_total;
-  int n_cams, lut_n, batch, max_steps, max_attempts, max_crossings, disk_enabled, pad;
-};
 
 struct Lane {
   RayC c;
@@ -64,15 +46,6 @@ struct Lane {
   int steps, attempts, ncross, nsamp;
   bool last_rej;
 };
-
-__device__ __forceinline__ int find_camera(const CamDev* cams, int n, long long gid) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (cams[mid].offset <= gid) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
 
 // Returns false for ids in a gap between camera record ranges (no record).
 template <bool VOL>
@@ -98,10 +71,6 @@ __device__ __forceinline__ bool lane_setup(Lane& L, const DevParams& P) {
   L.steps = L.attempts = L.ncross = L.nsamp = 0;
   L.last_rej = false;
   return true;
-}
-
-__device__ __forceinline__ bool in_support(const VolDev& v, double r, double u) {
-  return r >= v.sup_r_lo && r <= v.sup_r_hi && __builtin_fabs(u) <= v.sup_u_max;
 }
 
 // Volume transfer over the accepted step of length h ending at y.
@@ -228,29 +197,18 @@ __device__ __forceinline__ int step_dp45(Lane& L, const DevParams& P) {
 template <int INTEG, bool VOL>
 __global__ void __launch_bounds__(kBlock) k_trace_fp64(const DevParams* __restrict__ Pp) {
   const DevParams& P = *Pp;
-  const int lane = static_cast<int>(__lane_id());
-  const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   Lane L;
   L.gid = -1;
   bool done = false;
   for (;;) {
     const bool need = (L.gid < 0) && !done;
-    const unsigned long long m = __ballot(need);
-    if (m) {
-      const int leader = __ffsll(static_cast<long long>(m)) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(P.head, static_cast<unsigned long long>(__popcll(m)));
-      const unsigned int lo = __shfl(static_cast<unsigned int>(base & 0xffffffffull), leader);
-      const unsigned int hi = __shfl(static_cast<unsigned int>(base >> 32), leader);
-      base = (static_cast<unsigned long long>(hi) << 32) | lo;
-      if (need) {
-        const long long id = static_cast<long long>(base) + __popcll(m & below);
-        if (id < P.n_total) {
-          L.gid = id;
-          if (!lane_setup<VOL>(L, P)) L.gid = -1;
-        } else {
-          done = true;
-        }
+    const long long id = refill_id(need, P.head);
+    if (need) {
+      if (id < P.n_total) {
+        L.gid = id;
+        if (!lane_setup<VOL>(L, P)) L.gid = -1;
+      } else {
+        done = true;
       }
     }
     if (__ballot(!done) == 0ull) break;
@@ -376,8 +334,11 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
   const BH_TraceOptions& o = *opts;
   if (o.integrator != BH_INTEGRATOR_RK4 && o.integrator != BH_INTEGRATOR_DP45)
     return fail("bh_trace_image_plane: unknown integrator");
-  if (o.precision != BH_PRECISION_FP64)
-    return fail("bh_trace_image_plane: precision form not available in this build");
+  if (o.precision != BH_PRECISION_FP64 && o.precision != BH_PRECISION_FP32)
+    return fail("bh_trace_image_plane: unknown precision");
+  if (o.precision == BH_PRECISION_FP32 &&
+      !(o.island_r_factor >= 1.0 && o.island_turn_dist >= 0.0))
+    return fail("bh_trace_image_plane: FP32 form needs island_r_factor >= 1, island_turn_dist >= 0");
   if (o.integrator == BH_INTEGRATOR_RK4 && !(o.h_fixed > 0.0))
     return fail("bh_trace_image_plane: h_fixed must be > 0");
   if (o.integrator == BH_INTEGRATOR_DP45 &&
@@ -420,6 +381,7 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
     k.r_obs = c.r_obs;
     k.r_stop = o.horizon_factor * rh;
     k.r_isco = bh_isco_prograde(c.spin);
+    k.r_plus = rh;
     k.alpha_min = c.alpha_min;
     k.dalpha = (c.alpha_max - c.alpha_min) / static_cast<double>(c.width);
     k.beta_max = c.beta_max;
@@ -490,6 +452,21 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
   P.max_attempts = o.max_attempts;
   P.max_crossings = o.max_crossings;
   P.disk_enabled = o.disk_enabled;
+  P.island_r_factor = o.island_r_factor;
+  P.island_turn_dist = o.island_turn_dist;
+  P.rtol32 = static_cast<float>(std::max(o.rtol, 1e-6));
+  P.atol32 = static_cast<float>(std::max(o.atol, 1e-8));
+  {
+    const double nu = o.nu_obs;
+    const double me_c2 = kMElectron * kC * kC;
+    const double nuB_per_G = kECharge / (kTwoPi * kMElectron * kC);
+    P.CT = static_cast<float>(kH * nu / me_c2);
+    P.SB = static_cast<float>(2.0 * kH * nu * nu * nu / (kC * kC));
+    P.Cx = static_cast<float>(nu / (1.5 * nuB_per_G));
+    P.CjL = static_cast<float>(1.7320508075688772 * kECharge * kECharge * kECharge / me_c2 * nuB_per_G *
+                               0.5 * o.length_unit_cm);
+    P.Cd = static_cast<float>(o.disk_T_max > 0.0 ? kH * nu / (kKB * o.disk_T_max) : 1e30);
+  }
   std::memcpy(blk.data(), &P, sizeof(P));
   std::memcpy(blk.data() + sizeof(DevParams), hc.data(), hc.size() * sizeof(CamDev));
   // Pageable source: the runtime stages it before returning, so blk may die.
@@ -505,7 +482,9 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
   (void)hipEventRecord(g_ev0, s);
   const DevParams* dp = reinterpret_cast<const DevParams*>(ctx->d_block);
   hipError_t e;
-  if (o.integrator == BH_INTEGRATOR_DP45)
+  if (o.precision == BH_PRECISION_FP32)
+    e = launch_fp32(o.integrator, vol_on, dp, s, 0);
+  else if (o.integrator == BH_INTEGRATOR_DP45)
     e = vol_on ? launch<BH_INTEGRATOR_DP45, true>(dp, s, 0) : launch<BH_INTEGRATOR_DP45, false>(dp, s, 0);
   else
     e = vol_on ? launch<BH_INTEGRATOR_RK4, true>(dp, s, 0) : launch<BH_INTEGRATOR_RK4, false>(dp, s, 0);

@@ -57,3 +57,19 @@ def c4_cameras(n: int = 256, seed: int = 0x5EED0004):
         cams.append(camera(spin, inc, r_obs, side, side, 15.0))
     pack_cameras(cams)
     return cams
+
+
+def c3(width: int = 4096, height: int = 4096, grid=(256, 128, 64), seed: int = 0x5EED0003,
+       precision: int = abi.PRECISION_FP32):
+    """Precision form: a = 0.998, 85 deg, thin disk plus the c2 torus recipe,
+    FP32 DP45 stepping (rtol 1e-6, atol 1e-8) with FP64 islands at 230 GHz."""
+    cams = [camera(0.998, 85.0, 100.0, width, height, 20.0)]
+    opts = default_options(
+        integrator=abi.INTEGRATOR_DP45, precision=precision, max_steps=20000,
+        max_attempts=100000, disk_enabled=1, volume_enabled=1, rtol=1e-6, atol=1e-8,
+        h_init=0.01, h_max=0.1, h_vol=0.01, horizon_factor=1.01, r_escape=200.0,
+        disk_r_out=20.0, nu_obs=230.0e9, disk_T_max=1.0e7,
+        island_r_factor=1.2, island_turn_dist=0.05)
+    tp = torus_params(0.998, n_r=grid[0], n_th=grid[1], n_ph=grid[2], seed=seed)
+    cells, vol = generate_torus(tp)
+    return dict(cams=cams, opts=opts, luts=disk_luts(cams, 20.0), cells=cells, vol=vol, torus=tp)
