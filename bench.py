@@ -56,6 +56,7 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=1)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device copy (profiler counter calibration)")
     args = ap.parse_args()
@@ -73,6 +74,8 @@ def main() -> int:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
     w = workloads.c2(args.size, args.size)
+    if args.precision == "fp32":
+        w["opts"].precision = abi.PRECISION_FP32
     sc = tracer.upload_scene(w["cams"], w["opts"], luts=w["luts"], cells=w["cells"], vol=w["vol"])
     out = tracer.alloc_records(sc.n_records)
     stream = torch.cuda.current_stream()

@@ -1,0 +1,95 @@
+/*
+ * bhk_surrogate.h - C ABI of the learned geodesic surrogate (BF16 MFMA MLP).
+ *
+ * No counterpart in the reference (it has no learned component; SURVEY
+ * section 8(a) row a24).  The surrogate maps an image-plane ray
+ * (alpha, beta, a, cos theta_obs) to the geometry the thin-disk shading
+ * needs -- the first three equatorial crossings (radius, g), the crossing
+ * count, capture, the escape direction -- plus a confidence output that
+ * predicts its own error.  Rays with low confidence or inside the
+ * photon-ring band fall back to the integrator (bh_trace_image_plane with an
+ * id list); a random fraction is audited against the integrator.
+ *
+ * Network: 16 input features (an analytic embedding of the four inputs,
+ * padded to 32), 6 hidden layers of width 256 with SiLU, 16 outputs.  BF16
+ * weights and activations, FP32 accumulation on v_mfma_f32_16x16x32_bf16,
+ * FP32 master weights and Adam state for training.  All buffers device-side.
+ */
+#ifndef BHK_SURROGATE_H
+#define BHK_SURROGATE_H
+
+#include <stdint.h>
+
+#include "bhk_trace.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BH_SUR_FEATURES 16   /* embedded input features                      */
+#define BH_SUR_IN_PAD   32   /* feature row stride (MFMA K step)              */
+#define BH_SUR_OUT      16   /* output row stride                             */
+
+/* Output columns. */
+enum BH_SurrogateOutput {
+    BH_SUR_LOGR1 = 0, BH_SUR_G1 = 1,   /* first crossing: ln r, g              */
+    BH_SUR_LOGR2 = 2, BH_SUR_G2 = 3,
+    BH_SUR_LOGR3 = 4, BH_SUR_G3 = 5,
+    BH_SUR_NCROSS = 6,                 /* crossing count (0..3)                */
+    BH_SUR_CAPTURE = 7,                /* 1 captured, 0 escaped                */
+    BH_SUR_ESC_U = 8,                  /* cos(theta) at escape                 */
+    BH_SUR_ESC_SINPHI = 9,             /* sin / cos of phi at escape           */
+    BH_SUR_ESC_COSPHI = 10,
+    BH_SUR_CONF = 11                   /* log10 of the predicted squared error */
+};
+
+typedef struct BH_SurrogateConfig {
+    int32_t width;        /* hidden width (256)                                 */
+    int32_t depth;        /* hidden layers (6)                                  */
+    int32_t max_batch;    /* rows per forward / training call                   */
+    int32_t pad0;
+    uint64_t seed;        /* weight initialisation seed                         */
+} BH_SurrogateConfig;
+
+typedef struct BH_Surrogate BH_Surrogate; /* opaque, device-resident */
+
+/* Create / destroy.  Returns NULL on failure (bh_last_error()). */
+BH_Surrogate* bh_surrogate_create(const BH_SurrogateConfig* cfg);
+void bh_surrogate_destroy(BH_Surrogate* s);
+
+/* Parameters as one FP32 vector: for each layer l = 0..depth, W_l [out][in]
+ * (in = 32 for l = 0, out = 16 for l = depth) followed by b_l [out]. */
+int64_t bh_surrogate_param_count(const BH_Surrogate* s);
+int bh_surrogate_get_params(const BH_Surrogate* s, float* host_out);
+int bh_surrogate_set_params(BH_Surrogate* s, const float* host_in);
+
+/* Feature embedding of n rays given as (alpha, beta, spin, cos theta_obs)
+ * rows (device FP32 [n][4]) into device FP32 [n][BH_SUR_IN_PAD]. */
+int bh_surrogate_features(const float* rays4, int64_t n, float* features, void* stream);
+
+/* Forward pass: features (device FP32 [n][32]) -> outputs (device FP32
+ * [n][16]); n <= max_batch. */
+int bh_surrogate_forward(BH_Surrogate* s, const float* features, int n, float* out, void* stream);
+
+/* One training step (forward, weighted squared-error loss, backward, Adam)
+ * on a batch: targets and weights device FP32 [n][16]; loss_out (host,
+ * may be NULL) receives {sum w d^2 + conf term, sum w}. Synchronises. */
+int bh_surrogate_train_step(BH_Surrogate* s, const float* features, const float* targets,
+                            const float* weights, int n, float lr, float* loss_out, void* stream);
+
+/* Loss gradient with respect to every parameter (same layout as the
+ * parameter vector) without updating, for gradient checks.  Synchronises. */
+int bh_surrogate_gradients(BH_Surrogate* s, const float* features, const float* targets,
+                           const float* weights, int n, float* host_grads, void* stream);
+
+/* BF16 NT GEMM C[M][N] = A[M][K] B[N][K]^T (+ bias[N]) with FP32 output,
+ * K a multiple of 32 (the building block of the surrogate; exported for
+ * tests). */
+int bh_mlp_gemm_nt(int M, int N, int K, const void* A_bf16, int lda, const void* Bt_bf16, int ldb,
+                   const float* bias, float* C, int ldc, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BHK_SURROGATE_H */
