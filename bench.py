@@ -62,7 +62,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "rays are independent and shard without exchange, see DESIGN.md", file=sys.stderr)
+              "rays are independent and would shard by rows or cameras without exchange", file=sys.stderr)
         return 2
 
     import numpy as np

@@ -55,6 +55,13 @@ def lib():
         C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions),
         C.c_void_p, C.c_int, C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p,
     ]
+    L.bh_trace_pixel_list.argtypes = [
+        C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions), C.c_void_p, C.c_int,
+        C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+    L.bh_trace_pixel_list.restype = C.c_int
+    L.bh_trace_rays.argtypes = [C.c_void_p, C.c_int64, C.POINTER(abi.BH_TraceOptions),
+                                C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p]
+    L.bh_trace_rays.restype = C.c_int
     L.bh_last_kernel_ms.restype = C.c_float
     for name in (
         "bh_version", "bh_trace_options_default", "bh_disk_lut_build",

@@ -36,6 +36,11 @@ class BH_Camera(C.Structure):
     ]
 
 
+class BH_RaySpec(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("beta", C.c_double), ("spin", C.c_double),
+                ("theta_obs", C.c_double), ("r_obs", C.c_double)]
+
+
 class BH_TraceOptions(C.Structure):
     _fields_ = [
         ("integrator", C.c_int32),

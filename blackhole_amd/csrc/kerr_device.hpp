@@ -136,9 +136,7 @@ __device__ __forceinline__ void project_radial(const RayCT<T>& c, T* y, T* f) {
 }
 
 // Image plane -> constants and initial state (Bardeen 1973).
-__device__ __forceinline__ RayC setup_ray(const CamDev& k, int64_t px, int64_t py, double* y) {
-  const double alpha = k.alpha_min + (static_cast<double>(px) + 0.5) * k.dalpha;
-  const double beta = k.beta_max - (static_cast<double>(py) + 0.5) * k.dbeta;
+__device__ __forceinline__ RayC setup_ray_ab(const CamDev& k, double alpha, double beta, double* y) {
   const double a = k.a;
   const double xi = -(alpha * k.sin_o);
   const double eta = __builtin_fma(beta, beta, (alpha * alpha - a * a) * (k.cos_o * k.cos_o));
@@ -151,6 +149,11 @@ __device__ __forceinline__ RayC setup_ray(const CamDev& k, int64_t px, int64_t p
   y[3] = -__builtin_sqrt(R);
   y[4] = k.sin_o * beta;
   return c;
+}
+__device__ __forceinline__ RayC setup_ray(const CamDev& k, int64_t px, int64_t py, double* y) {
+  const double alpha = k.alpha_min + (static_cast<double>(px) + 0.5) * k.dalpha;
+  const double beta = k.beta_max - (static_cast<double>(py) + 0.5) * k.dbeta;
+  return setup_ray_ab(k, alpha, beta, y);
 }
 
 // Classic RK4 step from y with k1 = f(y).

@@ -58,6 +58,46 @@ __global__ void k_features(const float* __restrict__ rays4, long long n, float* 
   for (int k = BH_SUR_FEATURES; k < BH_SUR_IN_PAD; ++k) o[k] = 0.0f;
 }
 
+// Training targets and loss weights from traced records (disk disabled, so
+// every equatorial crossing is recorded; cross_g = 0 inside the ISCO).
+__global__ void k_targets(const BH_PixelRecord* __restrict__ rec, long long n, float* __restrict__ T,
+                          float* __restrict__ W, float w_r, float w_g, float w_n, float w_c, float w_e) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const BH_PixelRecord r = rec[i];
+  float* t = T + i * BH_SUR_OUT;
+  float* w = W + i * BH_SUR_OUT;
+#pragma unroll
+  for (int c = 0; c < BH_SUR_OUT; ++c) {
+    t[c] = 0.0f;
+    w[c] = 0.0f;
+  }
+  const int nc = r.n_cross;
+  for (int k = 0; k < 3; ++k) {
+    if (nc > k && r.cross_r[k] > 0.0) {
+      t[2 * k] = static_cast<float>(log(r.cross_r[k]));
+      w[2 * k] = w_r;
+      if (r.cross_g[k] > 0.0) {
+        t[2 * k + 1] = static_cast<float>(r.cross_g[k]);
+        w[2 * k + 1] = w_g;
+      }
+    }
+  }
+  t[BH_SUR_NCROSS] = static_cast<float>(nc < 3 ? nc : 3);
+  w[BH_SUR_NCROSS] = w_n;
+  const bool cap = r.hit_class == BH_HIT_HORIZON;
+  t[BH_SUR_CAPTURE] = cap ? 1.0f : 0.0f;
+  w[BH_SUR_CAPTURE] = (r.hit_class == BH_HIT_HORIZON || r.hit_class == BH_HIT_ESCAPED) ? w_c : 0.0f;
+  if (r.hit_class == BH_HIT_ESCAPED) {
+    double sp, cp;
+    sincos(r.phi_exit, &sp, &cp);
+    t[BH_SUR_ESC_U] = static_cast<float>(r.u_exit);
+    t[BH_SUR_ESC_SINPHI] = static_cast<float>(sp);
+    t[BH_SUR_ESC_COSPHI] = static_cast<float>(cp);
+    w[BH_SUR_ESC_U] = w[BH_SUR_ESC_SINPHI] = w[BH_SUR_ESC_COSPHI] = w_e;
+  }
+}
+
 }  // namespace bhk
 
 using namespace bhk;
@@ -277,6 +317,14 @@ int bh_surrogate_features(const float* rays4, int64_t n, float* features, void* 
   hipLaunchKernelGGL(k_features, grid1(n), dim3(256), 0, static_cast<hipStream_t>(stream), rays4,
                      static_cast<long long>(n), features);
   return hipGetLastError() == hipSuccess ? 0 : fail("bh_surrogate_features: launch failed");
+}
+
+int bh_surrogate_targets(const BH_PixelRecord* records, int64_t n, float* targets, float* weights,
+                         const float* w5, void* stream) {
+  if (!records || !targets || !weights || !w5 || n <= 0) return fail("bh_surrogate_targets: invalid arguments");
+  hipLaunchKernelGGL(k_targets, grid1(n), dim3(256), 0, static_cast<hipStream_t>(stream), records,
+                     static_cast<long long>(n), targets, weights, w5[0], w5[1], w5[2], w5[3], w5[4]);
+  return hipGetLastError() == hipSuccess ? 0 : fail("bh_surrogate_targets: launch failed");
 }
 
 int bh_surrogate_forward(BH_Surrogate* s, const float* features, int n, float* out, void* stream) {

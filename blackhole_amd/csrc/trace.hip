@@ -47,15 +47,14 @@ struct Lane {
   bool last_rej;
 };
 
-// Returns false for ids in a gap between camera record ranges (no record).
+// Sets up the ray of queue entry q; false for an id in a gap between camera
+// record ranges (no record).
 template <bool VOL>
-__device__ __forceinline__ bool lane_setup(Lane& L, const DevParams& P) {
-  L.cam = find_camera(P.cams, P.n_cams, L.gid);
-  const CamDev& k = P.cams[L.cam];
-  const long long local = L.gid - k.offset;
-  if (local >= k.w * k.h) return false;
-  const long long py = local / k.w, px = local - py * k.w;
-  L.c = setup_ray(k, px, py, L.y);
+__device__ __forceinline__ bool lane_setup(Lane& L, const DevParams& P, long long q) {
+  CamDev k;
+  double alpha, beta;
+  if (!resolve_ray(P, q, &L.gid, &L.cam, &k, &alpha, &beta)) return false;
+  L.c = setup_ray_ab(k, alpha, beta, L.y);
   rhs(L.c, L.y, L.k1);
   L.h = P.h_init;
   L.rt.I = 0.0;
@@ -200,13 +199,13 @@ __global__ void __launch_bounds__(kBlock) k_trace_fp64(const DevParams* __restri
   Lane L;
   L.gid = -1;
   bool done = false;
+  const long long n_queue = queue_length(P);
   for (;;) {
     const bool need = (L.gid < 0) && !done;
     const long long id = refill_id(need, P.head);
     if (need) {
-      if (id < P.n_total) {
-        L.gid = id;
-        if (!lane_setup<VOL>(L, P)) L.gid = -1;
+      if (id < n_queue) {
+        if (!lane_setup<VOL>(L, P, id)) L.gid = -1;
       } else {
         done = true;
       }
@@ -325,10 +324,20 @@ double bh_isco_prograde(double a) {
   return 3.0 + z2 - std::sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
 }
 
-int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
-                         const double* disk_luts, int disk_lut_n, const BH_Volume* vol,
-                         BH_PixelRecord* out, void* stream) {
-  if (!cams || n_cams <= 0) return fail("bh_trace_image_plane: no cameras");
+}  // extern "C"
+
+namespace bhk {
+
+// Shared launcher: every pixel of the cameras (rays == ids == NULL), a list
+// of pixel ids, or an explicit ray list.
+int trace_launch(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+                 const double* disk_luts, int disk_lut_n, const BH_Volume* vol,
+                 const BH_RaySpec* rays, long long n_rays, const int64_t* ids,
+                 const uint64_t* n_ids_dev, long long max_ids, BH_PixelRecord* out, void* stream) {
+  if (!rays && (!cams || n_cams <= 0)) return fail("bh_trace_image_plane: no cameras");
+  if (rays && n_rays <= 0) return fail("bh_trace_rays: empty ray list");
+  if (ids && (!n_ids_dev || max_ids <= 0)) return fail("bh_trace_pixel_list: invalid id list");
+  if (rays && opts && opts->disk_enabled) return fail("bh_trace_rays: the thin disk must be disabled");
   if (!opts) return fail("bh_trace_image_plane: NULL options");
   if (!out) return fail("bh_trace_image_plane: NULL output");
   const BH_TraceOptions& o = *opts;
@@ -359,9 +368,9 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
     if (o.integrator == BH_INTEGRATOR_DP45 && !(o.h_vol > 0.0))
       return fail("bh_trace_image_plane: h_vol must be > 0");
   }
-  std::vector<CamDev> hc(n_cams);
+  std::vector<CamDev> hc(rays ? 0 : n_cams);
   long long total = 0;
-  for (int i = 0; i < n_cams; ++i) {
+  for (int i = 0; i < (rays ? 0 : n_cams); ++i) {
     const BH_Camera& c = cams[i];
     if (!(c.spin >= 0.0 && c.spin < 1.0)) return fail("bh_trace_image_plane: spin outside [0, 1)");
     if (!(c.theta_obs > 0.0 && c.theta_obs < kPi))
@@ -444,8 +453,12 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
   P.luts = disk_luts;
   P.out = out;
   P.head = ctx->d_head;
-  P.n_total = total;
-  P.n_cams = n_cams;
+  P.n_total = rays ? n_rays : (ids ? max_ids : total);
+  P.rays = rays;
+  P.ids = reinterpret_cast<const long long*>(ids);
+  P.count_dev = reinterpret_cast<const unsigned long long*>(ids ? n_ids_dev : nullptr);
+  P.horizon_factor = o.horizon_factor;
+  P.n_cams = rays ? 0 : n_cams;
   P.lut_n = disk_lut_n;
   P.batch = o.refill_batch > 0 ? o.refill_batch : (o.integrator == BH_INTEGRATOR_DP45 ? 8 : 16);
   P.max_steps = o.max_steps;
@@ -492,6 +505,31 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
     return fail(std::string("bh_trace_image_plane: launch failed: ") + hipGetErrorString(e));
   (void)hipEventRecord(g_ev1, s);
   return 0;
+}
+
+}  // namespace bhk
+
+extern "C" {
+
+int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+                         const double* disk_luts, int disk_lut_n, const BH_Volume* vol,
+                         BH_PixelRecord* out, void* stream) {
+  return trace_launch(cams, n_cams, opts, disk_luts, disk_lut_n, vol, nullptr, 0, nullptr, nullptr, 0, out,
+                      stream);
+}
+
+int bh_trace_pixel_list(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+                        const double* disk_luts, int disk_lut_n, const BH_Volume* vol, const int64_t* ids,
+                        const uint64_t* n_ids_dev, int64_t max_ids, BH_PixelRecord* out, void* stream) {
+  if (!ids) return fail("bh_trace_pixel_list: NULL id list");
+  return trace_launch(cams, n_cams, opts, disk_luts, disk_lut_n, vol, nullptr, 0, ids, n_ids_dev, max_ids, out,
+                      stream);
+}
+
+int bh_trace_rays(const BH_RaySpec* rays, int64_t n, const BH_TraceOptions* opts, const BH_Volume* vol,
+                  BH_PixelRecord* out, void* stream) {
+  if (!rays) return fail("bh_trace_rays: NULL ray list");
+  return trace_launch(nullptr, 0, opts, nullptr, 0, vol, rays, n, nullptr, nullptr, 0, out, stream);
 }
 
 float bh_last_kernel_ms(void) {

@@ -25,6 +25,13 @@ struct DevParams {
   unsigned long long* head;  // ray-queue head
   long long n_total;
   int n_cams, lut_n, batch, max_steps, max_attempts, max_crossings, disk_enabled, pad;
+  // Ray sources other than every pixel of the cameras: an explicit ray list
+  // (rays, n_total entries, record i -> out[i]) or a list of pixel ids
+  // (ids, count read from count_dev, record -> out[id]).
+  const BH_RaySpec* rays;
+  const long long* ids;
+  const unsigned long long* count_dev;
+  double horizon_factor;
   // FP32 precision form: FP64 islands and FP32 tolerances.
   double island_r_factor, island_turn_dist;
   float rtol32, atol32;
@@ -42,6 +49,54 @@ __device__ __forceinline__ int find_camera(const CamDev* cams, int n, long long 
     if (cams[mid].offset <= gid) lo = mid; else hi = mid - 1;
   }
   return lo;
+}
+
+// Resolves queue entry q into the record index, the camera constants and the
+// image-plane point of its ray.  Returns false for a pixel id in a gap
+// between camera record ranges (no record).  cam = -1 for explicit rays.
+__device__ __forceinline__ bool resolve_ray(const DevParams& P, long long q, long long* rec,
+                                            int* cam, CamDev* k, double* alpha, double* beta) {
+  if (P.rays) {
+    const BH_RaySpec sp = P.rays[q];
+    const double a = sp.spin;
+    double s_o, c_o;
+    sincos(sp.theta_obs, &s_o, &c_o);
+    const double rp = 1.0 + __builtin_sqrt(1.0 - a * a);
+    const double z1 = 1.0 + cbrt(1.0 - a * a) * (cbrt(1.0 + a) + cbrt(1.0 - a));
+    const double z2 = __builtin_sqrt(3.0 * a * a + z1 * z1);
+    k->a = a;
+    k->sin_o = s_o;
+    k->cos_o = c_o;
+    k->r_obs = sp.r_obs;
+    k->r_plus = rp;
+    k->r_stop = P.horizon_factor * rp;
+    k->r_isco = 3.0 + z2 - __builtin_sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
+    k->lut_index = 0;
+    *alpha = sp.alpha;
+    *beta = sp.beta;
+    *rec = q;
+    *cam = -1;
+    return true;
+  }
+  const long long gid = P.ids ? P.ids[q] : q;
+  const int c = find_camera(P.cams, P.n_cams, gid);
+  *k = P.cams[c];
+  const long long local = gid - k->offset;
+  if (local < 0 || local >= k->w * k->h) return false;
+  const long long py = local / k->w, px = local - py * k->w;
+  *alpha = k->alpha_min + (static_cast<double>(px) + 0.5) * k->dalpha;
+  *beta = k->beta_max - (static_cast<double>(py) + 0.5) * k->dbeta;
+  *rec = gid;
+  *cam = c;
+  return true;
+}
+
+// Number of queue entries (an id-list count lives in device memory so that
+// a producer kernel can append ids without a host round trip).
+__device__ __forceinline__ long long queue_length(const DevParams& P) {
+  if (!P.count_dev) return P.n_total;
+  const long long c = static_cast<long long>(*P.count_dev);
+  return c < P.n_total ? c : P.n_total;  // never beyond the caller's max_ids
 }
 
 __device__ __forceinline__ bool in_support(const VolDev& v, double r, double u) {

@@ -47,14 +47,12 @@ __device__ __forceinline__ float affine_rate_f(const RayCT<float>& c, float r, f
 }
 
 template <bool VOL>
-__device__ __forceinline__ bool lane_setup(LaneF& L, const DevParams& P) {
-  L.cam = find_camera(P.cams, P.n_cams, L.gid);
-  const CamDev& k = P.cams[L.cam];
-  const long long local = L.gid - k.offset;
-  if (local >= k.w * k.h) return false;
-  const long long py = local / k.w, px = local - py * k.w;
+__device__ __forceinline__ bool lane_setup(LaneF& L, const DevParams& P, long long q) {
+  CamDev k;
+  double alpha, beta;
+  if (!resolve_ray(P, q, &L.gid, &L.cam, &k, &alpha, &beta)) return false;
   double yd[kN], fd[kN];
-  L.cd = setup_ray(k, px, py, yd);  // FP64 island: constants of motion
+  L.cd = setup_ray_ab(k, alpha, beta, yd);  // FP64 island: constants of motion
   L.cf = convert_const<float>(L.cd);
   rhs(L.cd, yd, fd);
 #pragma unroll
@@ -458,13 +456,13 @@ __global__ void __launch_bounds__(kBlock) k_trace_fp32(const DevParams* __restri
   LaneF L;
   L.gid = -1;
   bool done = false;
+  const long long n_queue = queue_length(P);
   for (;;) {
     const bool need = (L.gid < 0) && !done;
     const long long id = refill_id(need, P.head);
     if (need) {
-      if (id < P.n_total) {
-        L.gid = id;
-        if (!lane_setup<VOL>(L, P)) L.gid = -1;
+      if (id < n_queue) {
+        if (!lane_setup<VOL>(L, P, id)) L.gid = -1;
       } else {
         done = true;
       }

@@ -36,9 +36,11 @@ def _sigs(L):
     L.bh_surrogate_forward.argtypes = [vp, vp, C.c_int, vp, vp]
     L.bh_surrogate_train_step.argtypes = [vp, vp, vp, vp, C.c_int, C.c_float, fp, vp]
     L.bh_surrogate_gradients.argtypes = [vp, vp, vp, vp, C.c_int, fp, vp]
+    L.bh_surrogate_targets.argtypes = [vp, C.c_int64, vp, vp, fp, vp]
     L.bh_mlp_gemm_nt.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp]
     for n in ("bh_surrogate_get_params", "bh_surrogate_set_params", "bh_surrogate_features",
-              "bh_surrogate_forward", "bh_surrogate_train_step", "bh_surrogate_gradients", "bh_mlp_gemm_nt"):
+              "bh_surrogate_forward", "bh_surrogate_train_step", "bh_surrogate_gradients", "bh_mlp_gemm_nt",
+              "bh_surrogate_targets"):
         getattr(L, n).restype = C.c_int
     L._bhk_sur = True
     return L
@@ -128,6 +130,21 @@ class Surrogate:
         check(self.L.bh_surrogate_gradients(self.h, _ptr(feats), _ptr(targets), _ptr(weights), feats.shape[0],
                                             g.ctypes.data_as(C.POINTER(C.c_float)), _stream()), "gradients")
         return g
+
+
+LOSS_WEIGHTS = (1.0, 1.0, 0.1, 0.1, 0.05)  # ln r, g, count, capture, escape
+
+
+def targets_from_records(records_u8, weights=LOSS_WEIGHTS):
+    """Device targets / weights [n, 16] from a uint8 [n, 128] record tensor."""
+    import torch
+    n = records_u8.shape[0]
+    T = torch.empty((n, OUT), dtype=torch.float32, device=records_u8.device)
+    W = torch.empty_like(T)
+    w5 = (C.c_float * 5)(*weights)
+    L = _sigs(lib())
+    check(L.bh_surrogate_targets(_ptr(records_u8), n, _ptr(T), _ptr(W), w5, _stream()), "targets")
+    return T, W
 
 
 def gemm_nt(A_bf16, Bt_bf16, bias=None):

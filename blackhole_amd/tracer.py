@@ -151,6 +151,36 @@ def launch(sc: DeviceScene, out, stream=None) -> None:
     check(rc, "bh_trace_image_plane")
 
 
+def launch_ids(sc: DeviceScene, ids, count, out, stream=None) -> None:
+    """Asynchronous trace of the pixels listed in ids (int64 cuda) with the
+    list length in count (int64/uint64 cuda, one element), records to out[id]."""
+    torch = _torch()
+    cam_arr = (abi.BH_Camera * len(sc.cams))(*sc.cams)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    rc = lib().bh_trace_pixel_list(
+        cam_arr, len(sc.cams), C.byref(sc.opts),
+        C.c_void_p(sc.luts.data_ptr()) if sc.luts is not None else None, sc.lut_n,
+        C.byref(sc.vol) if sc.vol is not None else None,
+        C.c_void_p(ids.data_ptr()), C.c_void_p(count.data_ptr()), int(ids.numel()),
+        C.c_void_p(out.data_ptr()), C.c_void_p(s.cuda_stream))
+    check(rc, "bh_trace_pixel_list")
+
+
+def trace_rays(specs, opts, out=None, stream=None):
+    """Trace explicit rays: specs float64 cuda [n, 5] = (alpha, beta, spin,
+    theta_obs, r_obs); returns the uint8 [n, 128] record tensor."""
+    torch = _torch()
+    specs = specs.contiguous()
+    assert specs.dtype == torch.float64 and specs.shape[1] == 5
+    n = specs.shape[0]
+    if out is None:
+        out = alloc_records(n, device=specs.device)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib().bh_trace_rays(C.c_void_p(specs.data_ptr()), n, C.byref(opts), None,
+                              C.c_void_p(out.data_ptr()), C.c_void_p(s.cuda_stream)), "bh_trace_rays")
+    return out
+
+
 def records_to_numpy(out) -> np.ndarray:
     return out.cpu().numpy().view(abi.RECORD_DTYPE).reshape(-1)
 

@@ -1,8 +1,8 @@
 /*
  * bhk_surrogate.h - C ABI of the learned geodesic surrogate (BF16 MFMA MLP).
  *
- * No counterpart in the reference (it has no learned component; SURVEY
- * section 8(a) row a24).  The surrogate maps an image-plane ray
+ * No counterpart in the reference (it has no learned component).  The
+ * surrogate maps an image-plane ray
  * (alpha, beta, a, cos theta_obs) to the geometry the thin-disk shading
  * needs -- the first three equatorial crossings (radius, g), the crossing
  * count, capture, the escape direction -- plus a confidence output that
@@ -66,6 +66,13 @@ int bh_surrogate_set_params(BH_Surrogate* s, const float* host_in);
 /* Feature embedding of n rays given as (alpha, beta, spin, cos theta_obs)
  * rows (device FP32 [n][4]) into device FP32 [n][BH_SUR_IN_PAD]. */
 int bh_surrogate_features(const float* rays4, int64_t n, float* features, void* stream);
+
+/* Training targets and loss weights (device FP32 [n][16] each) from n traced
+ * records (disk disabled): ln r and g of the first three crossings, crossing
+ * count, capture, escape direction.  w5 (host) = loss weights of
+ * {ln r, g, count, capture, escape}. */
+int bh_surrogate_targets(const BH_PixelRecord* records, int64_t n, float* targets, float* weights,
+                         const float* w5, void* stream);
 
 /* Forward pass: features (device FP32 [n][32]) -> outputs (device FP32
  * [n][16]); n <= max_batch. */

@@ -92,6 +92,17 @@ typedef struct BH_Camera {
     int64_t record_offset;
 } BH_Camera; /* 72 bytes */
 
+/* One explicitly specified ray (a pixel of a 1x1 camera): image-plane
+ * impact parameters, spin, inclination and observer radius.  Used for
+ * training-set generation; per-ray sin/cos/ISCO are evaluated on the device
+ * (device libm), so these rays are not bit-identical to the camera path. */
+typedef struct BH_RaySpec {
+    double  alpha, beta;    /* impact parameters [M]                            */
+    double  spin;           /* a/M in [0, 1)                                    */
+    double  theta_obs;      /* inclination [rad], (0, pi)                       */
+    double  r_obs;          /* observer radius [M]                              */
+} BH_RaySpec; /* 40 bytes */
+
 /* Integration and radiative-transfer options (bh_trace_options_default). */
 typedef struct BH_TraceOptions {
     int32_t integrator;     /* BH_Integrator                                    */
@@ -223,6 +234,24 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cameras,
                          const double* disk_luts, int disk_lut_n,
                          const BH_Volume* vol,
                          BH_PixelRecord* out, void* stream);
+
+/* Trace only the pixels whose global record ids are listed (device int64
+ * ids[*n_ids_dev], count in device memory so a producer kernel can append
+ * ids without a host round trip; at most max_ids).  Record of id i goes to
+ * out[i].  Same cameras / options / tables as bh_trace_image_plane.  Used for
+ * the surrogate's fallback and audit rays. */
+int bh_trace_pixel_list(const BH_Camera* cams, int n_cameras,
+                        const BH_TraceOptions* opts,
+                        const double* disk_luts, int disk_lut_n,
+                        const BH_Volume* vol,
+                        const int64_t* ids, const uint64_t* n_ids_dev, int64_t max_ids,
+                        BH_PixelRecord* out, void* stream);
+
+/* Trace n explicitly specified rays (device BH_RaySpec[n]) into out[0..n)
+ * (device).  The thin disk must be disabled (crossings are still recorded);
+ * the volume may be enabled. */
+int bh_trace_rays(const BH_RaySpec* rays, int64_t n, const BH_TraceOptions* opts,
+                  const BH_Volume* vol, BH_PixelRecord* out, void* stream);
 
 /* Elapsed time of the last bh_trace_image_plane kernel on this thread, from
  * HIP events recorded around the launch on its stream (ms; -1 if none).  The
