@@ -203,6 +203,51 @@ __global__ void k_adam(long long n, float* __restrict__ p, const float* __restri
   if (p_bf) p_bf[i] = f2bf(p[i]);
 }
 
+// First layer in FP32 (the inputs are not quantised to BF16): Z0 = X W0^T + b0
+// over the n_feat live features, H1 = BF16(SiLU(Z0)).  One thread per
+// (row, output).  X [n][ldx], W0 [out][ldw], Z0 (optional) / H1 [n][out].
+__global__ void k_layer0_fwd(int n, int n_out, int n_feat, const float* __restrict__ X, int ldx,
+                             const float* __restrict__ W0, int ldw, const float* __restrict__ b0,
+                             float* __restrict__ Z0, uint16_t* __restrict__ H1) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<long long>(n) * n_out) return;
+  const int r = static_cast<int>(i / n_out), o = static_cast<int>(i - static_cast<long long>(r) * n_out);
+  const float* x = X + static_cast<size_t>(r) * ldx;
+  const float* w = W0 + static_cast<size_t>(o) * ldw;
+  float z = b0[o];
+  for (int k = 0; k < n_feat; ++k) z = fmaf(w[k], x[k], z);
+  if (Z0) Z0[i] = z;
+  H1[i] = f2bf(silu(z));
+}
+
+// dW0[o][k] = sum_r dZ0[r][o] X[r][k] for k < n_feat (FP32; pad columns 0).
+// One block per output o; 256 threads stride over rows.
+__global__ void k_layer0_wgrad(int n, int n_out, int n_feat, const uint16_t* __restrict__ dZ0, int ldz,
+                               const float* __restrict__ X, int ldx, float* __restrict__ dW0, int ldw) {
+  const int o = blockIdx.x;
+  float acc[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) acc[k] = 0.0f;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const float g = bf2f(dZ0[static_cast<size_t>(r) * ldz + o]);
+    const float* x = X + static_cast<size_t>(r) * ldx;
+#pragma unroll
+    for (int k = 0; k < 32; ++k)
+      if (k < n_feat) acc[k] = fmaf(g, x[k], acc[k]);
+  }
+  __shared__ float red[256];
+  for (int k = 0; k < ldw; ++k) {
+    red[threadIdx.x] = (k < 32) ? acc[k < 32 ? k : 0] : 0.0f;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) dW0[static_cast<size_t>(o) * ldw + k] = (k < n_feat) ? red[0] : 0.0f;
+    __syncthreads();
+  }
+}
+
 }  // namespace mlp
 }  // namespace bhk
 

@@ -20,7 +20,9 @@ int fail(const std::string& msg);
 
 // Analytic embedding of (alpha, beta, a, cos theta_obs) into 16 features
 // (padded with zeros to 32): scaled inputs, the Bardeen constants xi, eta,
-// polar coordinates of the image point and a few products.
+// polar coordinates of the image point, a few products and ln of the
+// flat-space equatorial crossing radius r_flat = sqrt(alpha^2 + (beta/cos)^2)
+// (feature 15 = ln(r_flat)/3; the ln r outputs are residuals against it).
 __host__ __device__ inline void surrogate_features(float alpha, float beta, float a, float cth,
                                                    float* f) {
   const float sth = sqrtf(fmaxf(1.0f - cth * cth, 0.0f));
@@ -43,7 +45,10 @@ __host__ __device__ inline void surrogate_features(float alpha, float beta, floa
   f[12] = a * sth;
   f[13] = 1.0f / (1.0f + rho * rho * 0.04f);
   f[14] = logf(1.0f + rho) * 0.333333333f;
-  f[15] = beta * cth * 0.05f;
+  // flat-space crossing of the equatorial plane: x = alpha, y = beta / cos(theta)
+  const float cc = fmaxf(cth, 0.05f);
+  const float rflat = sqrtf(alpha * alpha + (beta / cc) * (beta / cc));
+  f[15] = logf(fmaxf(rflat, 0.05f)) * 0.333333333f;
 }
 
 __global__ void k_features(const float* __restrict__ rays4, long long n, float* __restrict__ feat) {
@@ -60,8 +65,9 @@ __global__ void k_features(const float* __restrict__ rays4, long long n, float* 
 
 // Training targets and loss weights from traced records (disk disabled, so
 // every equatorial crossing is recorded; cross_g = 0 inside the ISCO).
-__global__ void k_targets(const BH_PixelRecord* __restrict__ rec, long long n, float* __restrict__ T,
-                          float* __restrict__ W, float w_r, float w_g, float w_n, float w_c, float w_e) {
+__global__ void k_targets(const BH_PixelRecord* __restrict__ rec, const float* __restrict__ feat, long long n,
+                          float* __restrict__ T, float* __restrict__ W, float w_r, float w_g, float w_n,
+                          float w_c, float w_e) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const BH_PixelRecord r = rec[i];
@@ -73,9 +79,10 @@ __global__ void k_targets(const BH_PixelRecord* __restrict__ rec, long long n, f
     w[c] = 0.0f;
   }
   const int nc = r.n_cross;
+  const double lnflat = 3.0 * static_cast<double>(feat[i * BH_SUR_IN_PAD + 15]);
   for (int k = 0; k < 3; ++k) {
     if (nc > k && r.cross_r[k] > 0.0) {
-      t[2 * k] = static_cast<float>(log(r.cross_r[k]));
+      t[2 * k] = static_cast<float>(log(r.cross_r[k]) - lnflat);
       w[2 * k] = w_r;
       if (r.cross_g[k] > 0.0) {
         t[2 * k + 1] = static_cast<float>(r.cross_g[k]);
@@ -110,6 +117,7 @@ struct BH_Surrogate {
   long long n_params = 0;
   float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
   uint16_t* d_pbf = nullptr;                 // BF16 mirror of the parameter vector
+  float* d_x0 = nullptr;                     // FP32 input features [B][32]
   std::vector<uint16_t*> d_wt;               // per layer BF16 W^T [in_pad][out_pad]
   std::vector<uint16_t*> d_x;                // activations X_l BF16 [B][in_pad_l], l = 0..L
   std::vector<float*> d_z;                   // pre-activations Z_l FP32 [B][width], l < L
@@ -141,10 +149,14 @@ void refresh_bf16(BH_Surrogate* s, hipStream_t st) {
   }
 }
 
-// Forward through all layers for n rows (features already BF16 in d_x[0]).
+// Forward through all layers for n rows (FP32 features in d_x0).  Layer 0
+// runs in FP32 on the VALU (inputs unquantised), layers 1..L on BF16 MFMA.
 // keep_z: store the pre-activations (training).
 void forward(BH_Surrogate* s, int n, bool keep_z, hipStream_t st) {
-  for (int l = 0; l < s->L; ++l) {
+  hipLaunchKernelGGL(k_layer0_fwd, grid1(static_cast<long long>(n) * s->out_dim[0]), dim3(256), 0, st, n,
+                     s->out_dim[0], BH_SUR_FEATURES, s->d_x0, BH_SUR_IN_PAD, s->d_p + s->w_off[0],
+                     BH_SUR_IN_PAD, s->d_p + s->b_off[0], keep_z ? s->d_z[0] : nullptr, s->d_x[1]);
+  for (int l = 1; l < s->L; ++l) {
     const int N = s->out_dim[l], K = s->in_pad[l];
     const dim3 g((N + 63) / 64, (n + 63) / 64);
     const uint16_t* W = s->d_pbf + s->w_off[l];
@@ -178,6 +190,14 @@ void backward(BH_Surrogate* s, int n, const float* T, const float* Wt, hipStream
       dZ = s->d_dz;
       ldz = W;
     }
+    if (l == 0) {
+      // FP32 first layer: dW0 from the FP32 features
+      hipLaunchKernelGGL(k_layer0_wgrad, dim3(N_out), dim3(256), 0, st, n, N_out, BH_SUR_FEATURES, dZ, ldz,
+                         s->d_x0, BH_SUR_IN_PAD, s->d_g + s->w_off[0], K_in);
+      hipLaunchKernelGGL(k_colsum_bf16, dim3((N_out + 63) / 64), dim3(256), 0, st, n, N_out, dZ, ldz,
+                         s->d_g + s->b_off[0]);
+      continue;
+    }
     // grad W_l [N_out][K_in] = dZ^T X_l : A = dZ^T [N_out][n], B = X_l^T [K_in][n]
     hipLaunchKernelGGL(k_transpose_bf16, dim3((N_out + 31) / 32, (n + 31) / 32), dim3(256), 0, st, n, N_out,
                        dZ, ldz, s->d_t1, n);
@@ -199,8 +219,9 @@ void backward(BH_Surrogate* s, int n, const float* T, const float* Wt, hipStream
 
 int load_features(BH_Surrogate* s, const float* features, int n, hipStream_t st) {
   if (!s || !features || n <= 0 || n > s->max_batch) return fail("surrogate: invalid batch");
-  hipLaunchKernelGGL(k_f32_to_bf16, grid1(static_cast<long long>(n) * BH_SUR_IN_PAD), dim3(256), 0, st,
-                     static_cast<long long>(n) * BH_SUR_IN_PAD, features, s->d_x[0]);
+  if (hipMemcpyAsync(s->d_x0, features, static_cast<size_t>(n) * BH_SUR_IN_PAD * sizeof(float),
+                     hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return fail("surrogate: feature copy failed");
   return 0;
 }
 
@@ -242,7 +263,7 @@ BH_Surrogate* bh_surrogate_create(const BH_SurrogateConfig* cfg) {
     ok = dalloc(&s->d_x[l], static_cast<size_t>(B) * (l == 0 ? BH_SUR_IN_PAD : W));
   s->d_z.resize(s->L - 1, nullptr);
   for (int l = 0; l < s->L - 1 && ok; ++l) ok = dalloc(&s->d_z[l], static_cast<size_t>(B) * W);
-  ok = ok && dalloc(&s->d_y, static_cast<size_t>(B) * BH_SUR_OUT) && dalloc(&s->d_dx, static_cast<size_t>(B) * W) &&
+  ok = ok && dalloc(&s->d_x0, static_cast<size_t>(B) * BH_SUR_IN_PAD) && dalloc(&s->d_y, static_cast<size_t>(B) * BH_SUR_OUT) && dalloc(&s->d_dx, static_cast<size_t>(B) * W) &&
        dalloc(&s->d_dz, static_cast<size_t>(B) * W) && dalloc(&s->d_dout, static_cast<size_t>(B) * 32) &&
        dalloc(&s->d_t1, static_cast<size_t>(B) * W) && dalloc(&s->d_t2, static_cast<size_t>(B) * W) &&
        dalloc(&s->d_loss, 2);
@@ -291,7 +312,7 @@ void bh_surrogate_destroy(BH_Surrogate* s) {
   for (auto p : s->d_wt) f(p);
   for (auto p : s->d_x) f(p);
   for (auto p : s->d_z) f(p);
-  f(s->d_y); f(s->d_dx); f(s->d_dz); f(s->d_dout); f(s->d_t1); f(s->d_t2); f(s->d_loss);
+  f(s->d_x0); f(s->d_y); f(s->d_dx); f(s->d_dz); f(s->d_dout); f(s->d_t1); f(s->d_t2); f(s->d_loss);
   delete s;
 }
 
@@ -319,10 +340,11 @@ int bh_surrogate_features(const float* rays4, int64_t n, float* features, void* 
   return hipGetLastError() == hipSuccess ? 0 : fail("bh_surrogate_features: launch failed");
 }
 
-int bh_surrogate_targets(const BH_PixelRecord* records, int64_t n, float* targets, float* weights,
-                         const float* w5, void* stream) {
-  if (!records || !targets || !weights || !w5 || n <= 0) return fail("bh_surrogate_targets: invalid arguments");
-  hipLaunchKernelGGL(k_targets, grid1(n), dim3(256), 0, static_cast<hipStream_t>(stream), records,
+int bh_surrogate_targets(const BH_PixelRecord* records, const float* features, int64_t n, float* targets,
+                         float* weights, const float* w5, void* stream) {
+  if (!records || !features || !targets || !weights || !w5 || n <= 0)
+    return fail("bh_surrogate_targets: invalid arguments");
+  hipLaunchKernelGGL(k_targets, grid1(n), dim3(256), 0, static_cast<hipStream_t>(stream), records, features,
                      static_cast<long long>(n), targets, weights, w5[0], w5[1], w5[2], w5[3], w5[4]);
   return hipGetLastError() == hipSuccess ? 0 : fail("bh_surrogate_targets: launch failed");
 }

@@ -36,7 +36,7 @@ def _sigs(L):
     L.bh_surrogate_forward.argtypes = [vp, vp, C.c_int, vp, vp]
     L.bh_surrogate_train_step.argtypes = [vp, vp, vp, vp, C.c_int, C.c_float, fp, vp]
     L.bh_surrogate_gradients.argtypes = [vp, vp, vp, vp, C.c_int, fp, vp]
-    L.bh_surrogate_targets.argtypes = [vp, C.c_int64, vp, vp, fp, vp]
+    L.bh_surrogate_targets.argtypes = [vp, vp, C.c_int64, vp, vp, fp, vp]
     L.bh_mlp_gemm_nt.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp]
     for n in ("bh_surrogate_get_params", "bh_surrogate_set_params", "bh_surrogate_features",
               "bh_surrogate_forward", "bh_surrogate_train_step", "bh_surrogate_gradients", "bh_mlp_gemm_nt",
@@ -132,18 +132,19 @@ class Surrogate:
         return g
 
 
-LOSS_WEIGHTS = (1.0, 1.0, 0.1, 0.1, 0.05)  # ln r, g, count, capture, escape
+LOSS_WEIGHTS = (1.0, 0.1, 0.1, 0.1, 0.05)  # ln r, g, count, capture, escape
 
 
-def targets_from_records(records_u8, weights=LOSS_WEIGHTS):
-    """Device targets / weights [n, 16] from a uint8 [n, 128] record tensor."""
+def targets_from_records(records_u8, feats, weights=LOSS_WEIGHTS):
+    """Device targets / weights [n, 16] from a uint8 [n, 128] record tensor and
+    the rays' features [n, 32] (ln r targets are residuals against ln r_flat)."""
     import torch
     n = records_u8.shape[0]
     T = torch.empty((n, OUT), dtype=torch.float32, device=records_u8.device)
     W = torch.empty_like(T)
     w5 = (C.c_float * 5)(*weights)
     L = _sigs(lib())
-    check(L.bh_surrogate_targets(_ptr(records_u8), n, _ptr(T), _ptr(W), w5, _stream()), "targets")
+    check(L.bh_surrogate_targets(_ptr(records_u8), _ptr(feats), n, _ptr(T), _ptr(W), w5, _stream()), "targets")
     return T, W
 
 

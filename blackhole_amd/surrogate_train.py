@@ -101,8 +101,8 @@ def build_dataset(n: int, seed: int, chunk: int = 1 << 22):
         j = min(n, i + chunk)
         d_specs = torch.from_numpy(specs[i:j]).cuda()
         rec = tracer.trace_rays(d_specs, opts)
-        T[i:j], W[i:j] = S.targets_from_records(rec)
         feats[i:j] = S.Surrogate.features(torch.from_numpy(rays4_from_specs(specs[i:j])).cuda())
+        T[i:j], W[i:j] = S.targets_from_records(rec, feats[i:j])
         del rec, d_specs
     torch.cuda.synchronize()
     return feats, T, W, specs

@@ -11,9 +11,11 @@
  * id list); a random fraction is audited against the integrator.
  *
  * Network: 16 input features (an analytic embedding of the four inputs,
- * padded to 32), 6 hidden layers of width 256 with SiLU, 16 outputs.  BF16
- * weights and activations, FP32 accumulation on v_mfma_f32_16x16x32_bf16,
- * FP32 master weights and Adam state for training.  All buffers device-side.
+ * padded to 32), 6 hidden layers of width 256 with SiLU, 16 outputs.  The
+ * 16 -> 256 input layer runs in FP32 (the inputs are not quantised); every
+ * hidden layer has BF16 weights and activations with FP32 accumulation on
+ * v_mfma_f32_16x16x32_bf16.  FP32 master weights and Adam state for
+ * training.  All buffers device-side.
  */
 #ifndef BHK_SURROGATE_H
 #define BHK_SURROGATE_H
@@ -68,11 +70,12 @@ int bh_surrogate_set_params(BH_Surrogate* s, const float* host_in);
 int bh_surrogate_features(const float* rays4, int64_t n, float* features, void* stream);
 
 /* Training targets and loss weights (device FP32 [n][16] each) from n traced
- * records (disk disabled): ln r and g of the first three crossings, crossing
- * count, capture, escape direction.  w5 (host) = loss weights of
+ * records (disk disabled) and their features: ln r of the first three
+ * crossings minus ln r_flat (feature 15 = ln(r_flat)/3), g, crossing count,
+ * capture, escape direction.  w5 (host) = loss weights of
  * {ln r, g, count, capture, escape}. */
-int bh_surrogate_targets(const BH_PixelRecord* records, int64_t n, float* targets, float* weights,
-                         const float* w5, void* stream);
+int bh_surrogate_targets(const BH_PixelRecord* records, const float* features, int64_t n,
+                         float* targets, float* weights, const float* w5, void* stream);
 
 /* Forward pass: features (device FP32 [n][32]) -> outputs (device FP32
  * [n][16]); n <= max_batch. */

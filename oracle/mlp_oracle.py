@@ -45,10 +45,11 @@ def silu_grad(z):
 
 def forward(params, shapes, X):
     layers = unpack(params, shapes)
-    h = bf16_round(X)
+    h = np.asarray(X, dtype=np.float64)  # the input layer runs in FP32 (unquantised)
     Zs, Hs = [], [h]
     for l, (W, b) in enumerate(layers):
-        z = h @ bf16_round(W).T + b
+        Wl = W if l == 0 else bf16_round(W)
+        z = h @ Wl.T + b
         if l < len(layers) - 1:
             Zs.append(z)
             h = silu(z)
@@ -76,7 +77,7 @@ def loss_and_grad(params, shapes, X, T, Wt):
         W, b = layers[l]
         grads.append((g.T @ Hs[l], g.sum(axis=0)))
         if l > 0:
-            g = (g @ bf16_round(W)) * silu_grad(Zs[l - 1])
+            g = (g @ (W if l == 0 else bf16_round(W))) * silu_grad(Zs[l - 1])
     grads.reverse()
     flat = np.concatenate([np.concatenate([gW.reshape(-1), gb]) for gW, gb in grads])
     return loss, flat
