@@ -59,6 +59,8 @@ def lib():
         C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions), C.c_void_p, C.c_int,
         C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
     L.bh_trace_pixel_list.restype = C.c_int
+    L.bh_trace_pixel_list_compact.argtypes = L.bh_trace_pixel_list.argtypes
+    L.bh_trace_pixel_list_compact.restype = C.c_int
     L.bh_trace_rays.argtypes = [C.c_void_p, C.c_int64, C.POINTER(abi.BH_TraceOptions),
                                 C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p]
     L.bh_trace_rays.restype = C.c_int

@@ -14,42 +14,10 @@
 #include "../../include/bhk_surrogate.h"
 
 #include "mlp_kernels.hpp"
+#include "sur_features.hpp"
 
 namespace bhk {
 int fail(const std::string& msg);
-
-// Analytic embedding of (alpha, beta, a, cos theta_obs) into 16 features
-// (padded with zeros to 32): scaled inputs, the Bardeen constants xi, eta,
-// polar coordinates of the image point, a few products and ln of the
-// flat-space equatorial crossing radius r_flat = sqrt(alpha^2 + (beta/cos)^2)
-// (feature 15 = ln(r_flat)/3; the ln r outputs are residuals against it).
-__host__ __device__ inline void surrogate_features(float alpha, float beta, float a, float cth,
-                                                   float* f) {
-  const float sth = sqrtf(fmaxf(1.0f - cth * cth, 0.0f));
-  const float rho = sqrtf(alpha * alpha + beta * beta);
-  const float xi = -alpha * sth;
-  const float eta = beta * beta + (alpha * alpha - a * a) * cth * cth;
-  const float inv = 1.0f / fmaxf(rho, 1e-6f);
-  f[0] = alpha * 0.05f;
-  f[1] = beta * 0.05f;
-  f[2] = a;
-  f[3] = cth;
-  f[4] = sth;
-  f[5] = xi * 0.05f;
-  f[6] = eta * 0.0025f;
-  f[7] = rho * 0.05f;
-  f[8] = (rho - 5.196152422706632f) * 0.2f;
-  f[9] = beta * inv;
-  f[10] = alpha * inv;
-  f[11] = a * cth;
-  f[12] = a * sth;
-  f[13] = 1.0f / (1.0f + rho * rho * 0.04f);
-  f[14] = logf(1.0f + rho) * 0.333333333f;
-  // flat-space crossing of the equatorial plane: x = alpha, y = beta / cos(theta)
-  const float cc = fmaxf(cth, 0.05f);
-  const float rflat = sqrtf(alpha * alpha + (beta / cc) * (beta / cc));
-  f[15] = logf(fmaxf(rflat, 0.05f)) * 0.333333333f;
-}
 
 __global__ void k_features(const float* __restrict__ rays4, long long n, float* __restrict__ feat) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -79,10 +47,10 @@ __global__ void k_targets(const BH_PixelRecord* __restrict__ rec, const float* _
     w[c] = 0.0f;
   }
   const int nc = r.n_cross;
-  const double lnflat = 3.0 * static_cast<double>(feat[i * BH_SUR_IN_PAD + 15]);
+  const double lnbase = 3.0 * static_cast<double>(feat[i * BH_SUR_IN_PAD + kSurBaseFeature]);
   for (int k = 0; k < 3; ++k) {
     if (nc > k && r.cross_r[k] > 0.0) {
-      t[2 * k] = static_cast<float>(log(r.cross_r[k]) - lnflat);
+      t[2 * k] = static_cast<float>(log(r.cross_r[k]) - lnbase);
       w[2 * k] = w_r;
       if (r.cross_g[k] > 0.0) {
         t[2 * k + 1] = static_cast<float>(r.cross_g[k]);
@@ -129,7 +97,29 @@ struct BH_Surrogate {
   uint16_t* d_t2 = nullptr;                  // transpose scratch [width][B]
   float* d_loss = nullptr;                   // 2 floats
   long long t = 0;                           // Adam step
+  void* d_render = nullptr;                  // rendering scratch (sur_infer.hip)
+  size_t render_cap = 0;
 };
+
+namespace bhk {
+
+// Device parameters of a surrogate for the fused rendering kernel.
+struct SurView {
+  const float* p32;
+  const uint16_t* pbf;
+  const long long* w_off;
+  const long long* b_off;
+  int width, L;
+  void** render_buf;
+  size_t* render_cap;
+};
+
+SurView surrogate_view(BH_Surrogate* s) {
+  return SurView{s->d_p, s->d_pbf, s->w_off.data(), s->b_off.data(), s->width, s->L, &s->d_render,
+                 &s->render_cap};
+}
+
+}  // namespace bhk
 
 namespace {
 
@@ -313,6 +303,7 @@ void bh_surrogate_destroy(BH_Surrogate* s) {
   for (auto p : s->d_x) f(p);
   for (auto p : s->d_z) f(p);
   f(s->d_x0); f(s->d_y); f(s->d_dx); f(s->d_dz); f(s->d_dout); f(s->d_t1); f(s->d_t2); f(s->d_loss);
+  f(s->d_render);
   delete s;
 }
 

@@ -330,10 +330,53 @@ namespace bhk {
 
 // Shared launcher: every pixel of the cameras (rays == ids == NULL), a list
 // of pixel ids, or an explicit ray list.
+// Per-camera device constants, with the checks every camera must pass
+// (spin in [0, 1), theta_obs in (0, pi), non-empty, increasing disjoint
+// record ranges, observer outside the horizon stop).  Returns -1 with the
+// error set (prefixed by who) on the first invalid camera.
+int make_cam_devs(const BH_Camera* cams, int n_cams, double horizon_factor, const char* who,
+                  std::vector<CamDev>* out, long long* total_out) {
+  const std::string w(who);
+  out->assign(n_cams, CamDev{});
+  long long total = 0;
+  for (int i = 0; i < n_cams; ++i) {
+    const BH_Camera& c = cams[i];
+    if (!(c.spin >= 0.0 && c.spin < 1.0)) return fail(w + ": spin outside [0, 1)");
+    if (!(c.theta_obs > 0.0 && c.theta_obs < kPi)) return fail(w + ": theta_obs outside (0, pi)");
+    if (c.width <= 0 || c.height <= 0) return fail(w + ": empty camera");
+    if (c.record_offset < 0) return fail(w + ": negative record_offset");
+    if (i > 0 && c.record_offset < cams[i - 1].record_offset +
+                                       static_cast<long long>(cams[i - 1].width) * cams[i - 1].height)
+      return fail(w + ": camera record ranges must be increasing and disjoint");
+    const double rh = bh_outer_horizon(c.spin);
+    if (!(c.r_obs > horizon_factor * rh)) return fail(w + ": r_obs inside the horizon stop");
+    CamDev& k = (*out)[i];
+    k.a = c.spin;
+    k.sin_o = std::sin(c.theta_obs);
+    k.cos_o = std::cos(c.theta_obs);
+    k.r_obs = c.r_obs;
+    k.r_stop = horizon_factor * rh;
+    k.r_isco = bh_isco_prograde(c.spin);
+    k.r_plus = rh;
+    k.alpha_min = c.alpha_min;
+    k.dalpha = (c.alpha_max - c.alpha_min) / static_cast<double>(c.width);
+    k.beta_max = c.beta_max;
+    k.dbeta = (c.beta_max - c.beta_min) / static_cast<double>(c.height);
+    k.w = c.width;
+    k.h = c.height;
+    k.offset = c.record_offset;
+    k.lut_index = i;
+    total = std::max(total, c.record_offset + static_cast<long long>(c.width) * c.height);
+  }
+  *total_out = total;
+  return 0;
+}
+
 int trace_launch(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
                  const double* disk_luts, int disk_lut_n, const BH_Volume* vol,
                  const BH_RaySpec* rays, long long n_rays, const int64_t* ids,
-                 const uint64_t* n_ids_dev, long long max_ids, BH_PixelRecord* out, void* stream) {
+                 const uint64_t* n_ids_dev, long long max_ids, BH_PixelRecord* out, void* stream,
+                 bool compact) {
   if (!rays && (!cams || n_cams <= 0)) return fail("bh_trace_image_plane: no cameras");
   if (rays && n_rays <= 0) return fail("bh_trace_rays: empty ray list");
   if (ids && (!n_ids_dev || max_ids <= 0)) return fail("bh_trace_pixel_list: invalid id list");
@@ -368,39 +411,9 @@ int trace_launch(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
     if (o.integrator == BH_INTEGRATOR_DP45 && !(o.h_vol > 0.0))
       return fail("bh_trace_image_plane: h_vol must be > 0");
   }
-  std::vector<CamDev> hc(rays ? 0 : n_cams);
+  std::vector<CamDev> hc;
   long long total = 0;
-  for (int i = 0; i < (rays ? 0 : n_cams); ++i) {
-    const BH_Camera& c = cams[i];
-    if (!(c.spin >= 0.0 && c.spin < 1.0)) return fail("bh_trace_image_plane: spin outside [0, 1)");
-    if (!(c.theta_obs > 0.0 && c.theta_obs < kPi))
-      return fail("bh_trace_image_plane: theta_obs outside (0, pi)");
-    if (c.width <= 0 || c.height <= 0) return fail("bh_trace_image_plane: empty camera");
-    if (c.record_offset < 0) return fail("bh_trace_image_plane: negative record_offset");
-    if (i > 0 && c.record_offset < cams[i - 1].record_offset +
-                                       static_cast<long long>(cams[i - 1].width) * cams[i - 1].height)
-      return fail("bh_trace_image_plane: camera record ranges must be increasing and disjoint");
-    const double rh = bh_outer_horizon(c.spin);
-    if (!(c.r_obs > o.horizon_factor * rh))
-      return fail("bh_trace_image_plane: r_obs inside the horizon stop");
-    CamDev& k = hc[i];
-    k.a = c.spin;
-    k.sin_o = std::sin(c.theta_obs);
-    k.cos_o = std::cos(c.theta_obs);
-    k.r_obs = c.r_obs;
-    k.r_stop = o.horizon_factor * rh;
-    k.r_isco = bh_isco_prograde(c.spin);
-    k.r_plus = rh;
-    k.alpha_min = c.alpha_min;
-    k.dalpha = (c.alpha_max - c.alpha_min) / static_cast<double>(c.width);
-    k.beta_max = c.beta_max;
-    k.dbeta = (c.beta_max - c.beta_min) / static_cast<double>(c.height);
-    k.w = c.width;
-    k.h = c.height;
-    k.offset = c.record_offset;
-    k.lut_index = i;
-    total = std::max(total, c.record_offset + static_cast<long long>(c.width) * c.height);
-  }
+  if (!rays && make_cam_devs(cams, n_cams, o.horizon_factor, "bh_trace_image_plane", &hc, &total) != 0) return -1;
   hipStream_t s = static_cast<hipStream_t>(stream);
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return fail("bh_trace_image_plane: no HIP device");
@@ -465,6 +478,7 @@ int trace_launch(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
   P.max_attempts = o.max_attempts;
   P.max_crossings = o.max_crossings;
   P.disk_enabled = o.disk_enabled;
+  P.compact_ids = (ids && compact) ? 1 : 0;
   P.island_r_factor = o.island_r_factor;
   P.island_turn_dist = o.island_turn_dist;
   P.rtol32 = static_cast<float>(std::max(o.rtol, 1e-6));
@@ -515,7 +529,7 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
                          const double* disk_luts, int disk_lut_n, const BH_Volume* vol,
                          BH_PixelRecord* out, void* stream) {
   return trace_launch(cams, n_cams, opts, disk_luts, disk_lut_n, vol, nullptr, 0, nullptr, nullptr, 0, out,
-                      stream);
+                      stream, false);
 }
 
 int bh_trace_pixel_list(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
@@ -523,13 +537,22 @@ int bh_trace_pixel_list(const BH_Camera* cams, int n_cams, const BH_TraceOptions
                         const uint64_t* n_ids_dev, int64_t max_ids, BH_PixelRecord* out, void* stream) {
   if (!ids) return fail("bh_trace_pixel_list: NULL id list");
   return trace_launch(cams, n_cams, opts, disk_luts, disk_lut_n, vol, nullptr, 0, ids, n_ids_dev, max_ids, out,
-                      stream);
+                      stream, false);
+}
+
+int bh_trace_pixel_list_compact(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+                                const double* disk_luts, int disk_lut_n, const BH_Volume* vol,
+                                const int64_t* ids, const uint64_t* n_ids_dev, int64_t max_ids,
+                                BH_PixelRecord* out, void* stream) {
+  if (!ids) return fail("bh_trace_pixel_list_compact: NULL id list");
+  return trace_launch(cams, n_cams, opts, disk_luts, disk_lut_n, vol, nullptr, 0, ids, n_ids_dev, max_ids, out,
+                      stream, true);
 }
 
 int bh_trace_rays(const BH_RaySpec* rays, int64_t n, const BH_TraceOptions* opts, const BH_Volume* vol,
                   BH_PixelRecord* out, void* stream) {
   if (!rays) return fail("bh_trace_rays: NULL ray list");
-  return trace_launch(nullptr, 0, opts, nullptr, 0, vol, rays, n, nullptr, nullptr, 0, out, stream);
+  return trace_launch(nullptr, 0, opts, nullptr, 0, vol, rays, n, nullptr, nullptr, 0, out, stream, false);
 }
 
 float bh_last_kernel_ms(void) {

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "kerr_device.hpp"
 
 namespace bhk {
@@ -24,7 +26,8 @@ struct DevParams {
   BH_PixelRecord* out;
   unsigned long long* head;  // ray-queue head
   long long n_total;
-  int n_cams, lut_n, batch, max_steps, max_attempts, max_crossings, disk_enabled, pad;
+  int n_cams, lut_n, batch, max_steps, max_attempts, max_crossings, disk_enabled;
+  int compact_ids;  // id list: record of entry q -> out[q] instead of out[id]
   // Ray sources other than every pixel of the cameras: an explicit ray list
   // (rays, n_total entries, record i -> out[i]) or a list of pixel ids
   // (ids, count read from count_dev, record -> out[id]).
@@ -86,7 +89,7 @@ __device__ __forceinline__ bool resolve_ray(const DevParams& P, long long q, lon
   const long long py = local / k->w, px = local - py * k->w;
   *alpha = k->alpha_min + (static_cast<double>(px) + 0.5) * k->dalpha;
   *beta = k->beta_max - (static_cast<double>(py) + 0.5) * k->dbeta;
-  *rec = gid;
+  *rec = P.compact_ids ? q : gid;
   *cam = c;
   return true;
 }
@@ -123,6 +126,11 @@ __device__ __forceinline__ long long refill_id(bool need, unsigned long long* he
   }
   return id;
 }
+
+// Host: device constants of n_cams cameras with their validation (trace.hip);
+// -1 with the error set, prefixed by who, on an invalid camera.
+int make_cam_devs(const BH_Camera* cams, int n_cams, double horizon_factor, const char* who,
+                  std::vector<CamDev>* out, long long* total);
 
 // FP32 precision form launcher (trace_fp32.hip).
 hipError_t launch_fp32(int integrator, bool vol_on, const DevParams* d_params, hipStream_t s,

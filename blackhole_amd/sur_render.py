@@ -1,0 +1,116 @@
+"""Image rendering with the surrogate plus integrator fallback and audit.
+
+One frame set (any number of cameras at r_obs = 100 M) goes through:
+  1. bh_surrogate_render: every pixel through the fused BF16-MFMA network
+     and decode (records, fallback id list, audit id list);
+  2. bh_trace_pixel_list: the fallback rays traced by the plain integrator
+     into the same image (records overwrite the surrogate's);
+  3. bh_trace_pixel_list_compact + bh_surrogate_audit: the audit sample
+     traced into a compact buffer and compared with the surrogate records.
+All steps are asynchronous on one stream; list lengths stay on the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi, tracer
+from . import surrogate as S
+from ._lib import check, lib
+
+STAT_NAMES = ("n", "n_cmp", "sum_rel", "max_rel", "class_diff", "sum_abs_dI", "sum_I_ref", "sum_dlnr")
+
+
+@dataclass
+class RenderConfig:
+    ring_band: float = 0.25     # M
+    edge_tol: float = 2e-3      # relative
+    conf_max: float = 0.0       # log10 of the predicted squared error (0: effectively off)
+    ambiguity: float = 0.3
+    audit_every: int = 100      # 1 % audit
+    intensity_floor_rel: float = 1e-6  # audit compares pixels with I_ref > floor x max I
+
+
+def ring_tables(cams) -> np.ndarray:
+    arr = (abi.BH_Camera * len(cams))(*cams)
+    out = np.zeros(len(cams) * S.RING_FLOATS, dtype=np.float32)
+    L = S._sigs(lib())
+    check(L.bh_surrogate_ring_tables(arr, len(cams), out.ctypes.data_as(C.POINTER(C.c_float))), "ring_tables")
+    return out.reshape(len(cams), S.RING_FLOATS)
+
+
+class SurrogateRenderer:
+    """Device buffers and launches of the surrogate + fallback pipeline."""
+
+    def __init__(self, sur: S.Surrogate, scene: tracer.DeviceScene, cfg: RenderConfig = RenderConfig(),
+                 fallback_frac: float = 1.0):
+        import torch
+        self.torch = torch
+        self.sur, self.sc, self.cfg = sur, scene, cfg
+        n = scene.n_records
+        self.n = n
+        dev = scene.luts.device
+        self.out = tracer.alloc_records(n, device=dev)
+        # the fallback list holds every ray by default (8 B per pixel), so it
+        # cannot overflow; the audit sample is 1 in audit_every of the rest
+        self.fb_cap = max(1024, int(n * fallback_frac))
+        self.audit_cap = max(4096, 2 * n // max(cfg.audit_every, 1) + 4096) if cfg.audit_every > 0 else 1024
+        self.fb_ids = torch.empty(self.fb_cap, dtype=torch.int64, device=dev)
+        self.audit_ids = torch.empty(self.audit_cap, dtype=torch.int64, device=dev)
+        self.counts = torch.zeros(2, dtype=torch.int64, device=dev)  # fallback, audit
+        self.audit_rec = tracer.alloc_records(self.audit_cap, device=dev)
+        self.stats = torch.zeros(8, dtype=torch.float64, device=dev)
+        self.ring = torch.from_numpy(ring_tables(scene.cams).reshape(-1)).to(dev)
+        self.params = S.RenderParams(cfg.ring_band, cfg.edge_tol, cfg.conf_max, cfg.ambiguity,
+                                     cfg.audit_every, 0)
+        self.floor_I = 0.0
+
+    def _stream(self):
+        return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+
+    def render(self) -> None:
+        """Surrogate pass only (records + lists)."""
+        cam_arr = (abi.BH_Camera * len(self.sc.cams))(*self.sc.cams)
+        L = S._sigs(lib())
+        p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+        check(L.bh_surrogate_render(
+            self.sur.h, cam_arr, len(self.sc.cams), C.byref(self.sc.opts), p(self.sc.luts), self.sc.lut_n,
+            p(self.ring), C.byref(self.params), p(self.out), p(self.fb_ids), p(self.counts[0:1]),
+            self.fb_cap, p(self.audit_ids), p(self.counts[1:2]), self.audit_cap, self._stream()),
+            "bh_surrogate_render")
+
+    def fallback(self) -> None:
+        tracer.launch_ids(self.sc, self.fb_ids, self.counts[0:1], self.out)
+
+    def audit(self) -> None:
+        tracer.launch_ids(self.sc, self.audit_ids, self.counts[1:2], self.audit_rec, compact=True)
+        L = S._sigs(lib())
+        p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+        check(L.bh_surrogate_audit(p(self.out), p(self.audit_ids), p(self.counts[1:2]), self.audit_cap,
+                                   p(self.audit_rec), self.floor_I, p(self.stats), self._stream()),
+              "bh_surrogate_audit")
+
+    def run(self, audit: bool = True) -> None:
+        self.render()
+        self.fallback()
+        if audit:
+            self.audit()
+
+    def counts_host(self) -> tuple[int, int]:
+        """(fallback, audit) list lengths; raises if a list overflowed (its
+        rays would then be missing from the fallback or the audit)."""
+        c = self.counts.cpu().numpy()
+        fb, au = int(c[0]), int(c[1])
+        if fb > self.fb_cap or au > self.audit_cap:
+            raise RuntimeError(f"surrogate render: list overflow (fallback {fb} > {self.fb_cap} "
+                               f"or audit {au} > {self.audit_cap})")
+        return fb, au
+
+    def audit_stats(self) -> dict:
+        s = self.stats.cpu().numpy()
+        d = dict(zip(STAT_NAMES, (float(x) for x in s)))
+        d["mean_rel"] = d["sum_rel"] / max(d["n_cmp"], 1.0)
+        d["flux_weighted"] = d["sum_abs_dI"] / max(d["sum_I_ref"], 1e-300)
+        return d

@@ -16,6 +16,16 @@ FEATURES, IN_PAD, OUT = 16, 32, 16
 (LOGR1, G1, LOGR2, G2, LOGR3, G3, NCROSS, CAPTURE, ESC_U, ESC_SINPHI, ESC_COSPHI, CONF) = range(12)
 
 
+RING_BINS = 256
+RING_FLOATS = 2 + 3 * RING_BINS
+
+
+class RenderParams(C.Structure):
+    """BH_SurrogateRenderParams (fallback criteria and audit rate)."""
+    _fields_ = [("ring_band", C.c_float), ("edge_tol", C.c_float), ("conf_max", C.c_float),
+                ("ambiguity", C.c_float), ("audit_every", C.c_int32), ("pad0", C.c_int32)]
+
+
 class SurrogateConfig(C.Structure):
     _fields_ = [("width", C.c_int32), ("depth", C.c_int32), ("max_batch", C.c_int32),
                 ("pad0", C.c_int32), ("seed", C.c_uint64)]
@@ -38,7 +48,14 @@ def _sigs(L):
     L.bh_surrogate_gradients.argtypes = [vp, vp, vp, vp, C.c_int, fp, vp]
     L.bh_surrogate_targets.argtypes = [vp, vp, C.c_int64, vp, vp, fp, vp]
     L.bh_mlp_gemm_nt.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp]
-    for n in ("bh_surrogate_get_params", "bh_surrogate_set_params", "bh_surrogate_features",
+    from . import abi
+    L.bh_surrogate_ring_tables.argtypes = [C.POINTER(abi.BH_Camera), C.c_int, fp]
+    L.bh_surrogate_render.argtypes = [vp, C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions),
+                                      vp, C.c_int, vp, C.POINTER(RenderParams), vp, vp, vp, C.c_int64,
+                                      vp, vp, C.c_int64, vp]
+    L.bh_surrogate_audit.argtypes = [vp, vp, vp, C.c_int64, vp, C.c_double, vp, vp]
+    for n in ("bh_surrogate_ring_tables", "bh_surrogate_render", "bh_surrogate_audit",
+              "bh_surrogate_get_params", "bh_surrogate_set_params", "bh_surrogate_features",
               "bh_surrogate_forward", "bh_surrogate_train_step", "bh_surrogate_gradients", "bh_mlp_gemm_nt",
               "bh_surrogate_targets"):
         getattr(L, n).restype = C.c_int
@@ -137,7 +154,8 @@ LOSS_WEIGHTS = (1.0, 0.1, 0.1, 0.1, 0.05)  # ln r, g, count, capture, escape
 
 def targets_from_records(records_u8, feats, weights=LOSS_WEIGHTS):
     """Device targets / weights [n, 16] from a uint8 [n, 128] record tensor and
-    the rays' features [n, 32] (ln r targets are residuals against ln r_flat)."""
+    the rays' features [n, 32] (ln r targets are residuals against 3 x feature 14,
+    the logarithm of the light-bending estimate of the first crossing radius)."""
     import torch
     n = records_u8.shape[0]
     T = torch.empty((n, OUT), dtype=torch.float32, device=records_u8.device)

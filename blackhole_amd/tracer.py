@@ -151,19 +151,21 @@ def launch(sc: DeviceScene, out, stream=None) -> None:
     check(rc, "bh_trace_image_plane")
 
 
-def launch_ids(sc: DeviceScene, ids, count, out, stream=None) -> None:
+def launch_ids(sc: DeviceScene, ids, count, out, stream=None, compact: bool = False) -> None:
     """Asynchronous trace of the pixels listed in ids (int64 cuda) with the
-    list length in count (int64/uint64 cuda, one element), records to out[id]."""
+    list length in count (int64/uint64 cuda, one element), records to out[id]
+    (compact: list entry q to out[q])."""
     torch = _torch()
     cam_arr = (abi.BH_Camera * len(sc.cams))(*sc.cams)
     s = stream if stream is not None else torch.cuda.current_stream()
-    rc = lib().bh_trace_pixel_list(
+    fn = lib().bh_trace_pixel_list_compact if compact else lib().bh_trace_pixel_list
+    rc = fn(
         cam_arr, len(sc.cams), C.byref(sc.opts),
         C.c_void_p(sc.luts.data_ptr()) if sc.luts is not None else None, sc.lut_n,
         C.byref(sc.vol) if sc.vol is not None else None,
         C.c_void_p(ids.data_ptr()), C.c_void_p(count.data_ptr()), int(ids.numel()),
         C.c_void_p(out.data_ptr()), C.c_void_p(s.cuda_stream))
-    check(rc, "bh_trace_pixel_list")
+    check(rc, "bh_trace_pixel_list_compact" if compact else "bh_trace_pixel_list")
 
 
 def trace_rays(specs, opts, out=None, stream=None):

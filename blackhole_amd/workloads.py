@@ -11,7 +11,10 @@ c2  DP45 (rtol 1e-8, atol 1e-10) plus volumetric thermal synchrotron at
 c3  precision form: a = 0.998, 85 deg, 4096^2, disk + the c2 torus recipe.
 c4  batched: 256 cameras (N ~ logU(64, 1024), a ~ U[0, 0.998],
     theta ~ U[5, 85] deg, r_obs ~ U[50, 1000]) from a fixed seed.
-c5  surrogate evaluation frames: 2048^2 over alpha, beta in [-20, 20].
+c5  surrogate evaluation frames: 2048^2 over alpha, beta in [-20, 20] from
+    r_obs = 100 on a 4 x 4 sweep a in {0, 0.5, 0.9, 0.998} x theta in
+    {17, 40, 60, 85} deg; the c1 disk (NT, r_isco -> 20 M, 1 keV, T_max
+    1e7 K) traced by the plain DP45 integrator at the c2 tolerances.
 """
 from __future__ import annotations
 
@@ -73,3 +76,22 @@ def c3(width: int = 4096, height: int = 4096, grid=(256, 128, 64), seed: int = 0
     tp = torus_params(0.998, n_r=grid[0], n_th=grid[1], n_ph=grid[2], seed=seed)
     cells, vol = generate_torus(tp)
     return dict(cams=cams, opts=opts, luts=disk_luts(cams, 20.0), cells=cells, vol=vol, torus=tp)
+
+
+C5_SPINS = (0.0, 0.5, 0.9, 0.998)
+C5_INCLINATIONS = (17.0, 40.0, 60.0, 85.0)
+
+
+def c5_options():
+    """Plain DP45 FP64 thin-disk tracing (the surrogate's reference and fallback)."""
+    return default_options(
+        integrator=abi.INTEGRATOR_DP45, precision=abi.PRECISION_FP64, max_steps=20000,
+        max_attempts=100000, disk_enabled=1, volume_enabled=0, rtol=1e-8, atol=1e-10,
+        h_init=0.01, h_max=0.1, horizon_factor=1.01, r_escape=200.0, disk_r_out=20.0,
+        nu_obs=KEV_HZ, disk_T_max=1.0e7, max_crossings=3)
+
+
+def c5(size: int = 2048, spins=C5_SPINS, inclinations=C5_INCLINATIONS):
+    cams = [camera(a, th, 100.0, size, size, 20.0) for a in spins for th in inclinations]
+    pack_cameras(cams)
+    return dict(cams=cams, opts=c5_options(), luts=disk_luts(cams, 20.0), cells=None, vol=None)

@@ -71,8 +71,9 @@ int bh_surrogate_features(const float* rays4, int64_t n, float* features, void* 
 
 /* Training targets and loss weights (device FP32 [n][16] each) from n traced
  * records (disk disabled) and their features: ln r of the first three
- * crossings minus ln r_flat (feature 15 = ln(r_flat)/3), g, crossing count,
- * capture, escape direction.  w5 (host) = loss weights of
+ * crossings minus 3 x feature 14 (the logarithm of the light-bending
+ * estimate of the first crossing radius for an observer at r_obs = 100 M),
+ * g, crossing count, capture, escape direction.  w5 (host) = loss weights of
  * {ln r, g, count, capture, escape}. */
 int bh_surrogate_targets(const BH_PixelRecord* records, const float* features, int64_t n,
                          float* targets, float* weights, const float* w5, void* stream);
@@ -91,6 +92,53 @@ int bh_surrogate_train_step(BH_Surrogate* s, const float* features, const float*
  * parameter vector) without updating, for gradient checks.  Synchronises. */
 int bh_surrogate_gradients(BH_Surrogate* s, const float* features, const float* targets,
                            const float* weights, int n, float* host_grads, void* stream);
+
+/* ---- image rendering with the surrogate ------------------------------------ */
+
+/* Per-camera polar table of the critical curve (the photon-ring edge) used for
+ * the fallback band: BH_SUR_RING_FLOATS floats per camera =
+ * {alpha_c, beta_c, lo[256], hi[256], obl[256]}: centre, the curve's smallest
+ * and largest radius about the centre in each of 256 polar bins, and
+ * 1 / cos of the largest angle between the radial direction and the curve's
+ * normal in the bin (converts a distance band into a radial band). */
+#define BH_SUR_RING_BINS 256
+#define BH_SUR_RING_FLOATS (2 + 3 * BH_SUR_RING_BINS)
+int bh_surrogate_ring_tables(const BH_Camera* cams, int n_cams, float* host_out);
+
+typedef struct BH_SurrogateRenderParams {
+    float ring_band;      /* M: rays closer than this to the critical curve fall back */
+    float edge_tol;       /* relative: crossings this close to r_ISCO / r_out fall back */
+    float conf_max;       /* fall back when the confidence output exceeds this      */
+    float ambiguity;      /* fall back when count / capture outputs are this far from
+                             an integer                                              */
+    int32_t audit_every;  /* 1-in-N deterministic audit sample (0: none)            */
+    int32_t pad0;
+} BH_SurrogateRenderParams;
+
+/* Render every pixel of the cameras with the fused surrogate kernel
+ * (features, 7 layers and the decode in one launch; width 256 only): the
+ * thin-disk record (hit class, first crossings, g from the exact Keplerian
+ * formula at the predicted radius, intensity from the disk LUT, flags
+ * BH_FLAG_SURROGATE and, for rays that must be traced, BH_FLAG_FALLBACK).
+ * Ids of fallback rays are appended to fallback_ids (count in device memory,
+ * zeroed by the call), ids of a deterministic 1-in-audit_every sample of the
+ * other rays to audit_ids; both lists are in the form bh_trace_pixel_list
+ * takes.  Cameras must have r_obs = 100 M (the surrogate's training
+ * distance).  All pointers except cams, opts, rp are device memory;
+ * ring_tables from bh_surrogate_ring_tables, uploaded. */
+int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+                        const double* disk_luts, int disk_lut_n, const float* ring_tables,
+                        const BH_SurrogateRenderParams* rp, BH_PixelRecord* out,
+                        int64_t* fallback_ids, uint64_t* fallback_count, int64_t fallback_cap,
+                        int64_t* audit_ids, uint64_t* audit_count, int64_t audit_cap, void* stream);
+
+/* Audit statistics of the surrogate records out[ids[q]] against integrator
+ * records traced[q] (bh_trace_pixel_list_compact of the same ids), over
+ * q < min(*count, cap).  stats (device, 8 doubles, zeroed by the call):
+ * {n, n_cmp (reference intensity > floor), sum |dI|/I, max |dI|/I,
+ *  class disagreements, sum |dI|, sum I_ref, sum |d ln r_1| (both disk hits)}. */
+int bh_surrogate_audit(const BH_PixelRecord* out, const int64_t* ids, const uint64_t* count, int64_t cap,
+                       const BH_PixelRecord* traced, double floor_I, double* stats, void* stream);
 
 /* BF16 NT GEMM C[M][N] = A[M][K] B[N][K]^T (+ bias[N]) with FP32 output,
  * K a multiple of 32 (the building block of the surrogate; exported for

@@ -247,6 +247,16 @@ int bh_trace_pixel_list(const BH_Camera* cams, int n_cameras,
                         const int64_t* ids, const uint64_t* n_ids_dev, int64_t max_ids,
                         BH_PixelRecord* out, void* stream);
 
+/* As bh_trace_pixel_list, but the record of list entry q goes to out[q]
+ * (out holds max_ids records): the audit traces a sample of pixels into a
+ * compact buffer beside the full image it checks. */
+int bh_trace_pixel_list_compact(const BH_Camera* cams, int n_cameras,
+                                const BH_TraceOptions* opts,
+                                const double* disk_luts, int disk_lut_n,
+                                const BH_Volume* vol,
+                                const int64_t* ids, const uint64_t* n_ids_dev, int64_t max_ids,
+                                BH_PixelRecord* out, void* stream);
+
 /* Trace n explicitly specified rays (device BH_RaySpec[n]) into out[0..n)
  * (device).  The thin disk must be disabled (crossings are still recorded);
  * the volume may be enabled. */

@@ -84,3 +84,142 @@ def test_surrogate_forward_matches_checker_and_training_reduces_loss():
         last = sur.train_step(feats, Tt, Wt, 1e-3)[0]
     print("loss", first, "->", last, flush=True)
     assert last < 0.05 * first
+
+
+def test_fused_render_matches_layerwise_forward_checker_and_decode():
+    """The fused render kernel's raw network outputs (confidence output and the
+    first ln r output, recovered from the records) against the layer-by-layer
+    forward and the FP64 checker; the decode (g, intensity, flags, lists)
+    against the oracle's disk formulas on the host."""
+    import torch
+    import mlp_oracle
+    import oracle_lib as OL
+    from blackhole_amd import abi, analysis, sur_render, tracer
+    from blackhole_amd import surrogate as S
+    sur = S.Surrogate(width=256, depth=6, max_batch=4096, seed=21)
+    cams = [tracer.camera(0.3, 60.0, 100.0, 48, 40, 20.0), tracer.camera(0.9, 17.0, 100.0, 32, 32, 20.0)]
+    cams[0].record_offset = 0
+    cams[1].record_offset = 48 * 40 + 100  # a gap of 100 records
+    n = cams[1].record_offset + 32 * 32
+    opts = tracer.default_options(integrator=abi.INTEGRATOR_DP45, disk_enabled=1, disk_r_out=20.0,
+                                  nu_obs=2.417989242e17, disk_T_max=1.0e7)
+    luts = tracer.disk_luts(cams, 20.0)
+    sc = tracer.upload_scene(cams, opts, luts)
+    R = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(ring_band=0.25, audit_every=7))
+    R.out.fill_(0xAB)
+    R.render()
+    torch.cuda.synchronize()
+    rec = tracer.records_to_numpy(R.out)
+    gap = R.out[48 * 40:48 * 40 + 100].cpu().numpy()
+    assert (gap == 0xAB).all(), "records in the gap between cameras were written"
+    rays, ids = [], []
+    for c in cams:
+        A, B = analysis.pixel_grid(c)
+        m = A.size
+        rays.append(np.stack([A.reshape(-1), B.reshape(-1), np.full(m, c.spin), np.full(m, np.cos(c.theta_obs))], 1))
+        ids.append(c.record_offset + np.arange(m))
+    rays = np.concatenate(rays).astype(np.float32)
+    ids = np.concatenate(ids)
+    feats = S.Surrogate.features(torch.from_numpy(rays).cuda())
+    y_layer = sur.forward(feats).cpu().numpy()
+    F = feats.cpu().numpy().astype(np.float64)
+    y_ref, _, _ = mlp_oracle.forward(sur.get_params().astype(np.float64), sur.layer_shapes(), F)
+    r = rec[ids]
+    assert ((r["flags"] & abi.FLAG_SURROGATE) != 0).all()
+    conf = r["confidence"].astype(np.float64)
+    scale = np.abs(y_ref[:, S.CONF]).max() + 1e-3
+    print("conf fused-ref", np.abs(conf - y_ref[:, S.CONF]).max(), "layer-ref",
+          np.abs(y_layer[:, S.CONF] - y_ref[:, S.CONF]).max(), "scale", scale, flush=True)
+    assert np.abs(conf - y_ref[:, S.CONF]).max() <= 0.05 * scale
+    assert np.abs(conf - y_layer[:, S.CONF]).max() <= 0.05 * scale
+    has = r["n_cross"] >= 1
+    lnr = np.log(r["cross_r"][has, 0]) - 3.0 * F[has, 14]
+    sc0 = np.abs(y_ref[:, S.LOGR1]).max() + 1e-3
+    assert has.sum() > 0
+    assert np.abs(lnr - y_ref[has, S.LOGR1]).max() <= 0.05 * sc0
+    # decode: intensity of disk records from the oracle's disk formulas
+    L = OL.lib()
+    for k, cam in enumerate(cams):
+        m = (ids >= cam.record_offset) & (ids < cam.record_offset + cam.width * cam.height)
+        rr = r[m]
+        r_isco = tracer.isco(cam.spin)
+        lut = np.ascontiguousarray(luts[k])
+        A, B = analysis.pixel_grid(cam)
+        xi = -A.reshape(-1) * np.sin(cam.theta_obs)
+        for q in np.nonzero(rr["hit_class"] == abi.HIT_DISK)[0][:200]:
+            c = int(rr["n_cross"][q]) - 1
+            rc = rr["cross_r"][q, c]
+            assert r_isco <= rc <= 20.0
+            g = L.orc_disk_g(rc, cam.spin, xi[q])
+            assert abs(rr["cross_g"][q, c] - g) <= 1e-12 * abs(g)
+            Fv = L.orc_lut_linear(OL.dptr(lut), lut.size, r_isco, 20.0, rc)
+            I = L.orc_disk_intensity(Fv, g, opts.nu_obs, opts.disk_T_max)
+            assert abs(rr["intensity"][q] - I) <= 1e-9 * abs(I) + 1e-300
+    # lists: fallback ids are exactly the flagged records; audit ids are unflagged
+    fb, au = R.counts_host()
+    fb_ids = np.sort(R.fb_ids[:fb].cpu().numpy())
+    flagged = np.sort(ids[(r["flags"] & abi.FLAG_FALLBACK) != 0])
+    assert np.array_equal(fb_ids, flagged)
+    au_ids = R.audit_ids[:au].cpu().numpy()
+    assert au > 0 and ((rec[au_ids]["flags"] & abi.FLAG_FALLBACK) == 0).all()
+    assert len(np.unique(au_ids)) == au
+
+
+def test_render_pipeline_fallback_rays_match_integrator():
+    """Fallback rays carry the integrator's records bit for bit; the audit
+    traces its sample into a compact buffer identical to the full trace."""
+    import torch
+    from blackhole_amd import abi, sur_render, tracer, workloads
+    from blackhole_amd import surrogate as S
+    sur = S.Surrogate(width=256, depth=6, max_batch=4096, seed=5)
+    W = workloads.c5(64, spins=(0.5, 0.9), inclinations=(30.0,))
+    sc = tracer.upload_scene(W["cams"], W["opts"], W["luts"])
+    plain = tracer.alloc_records(sc.n_records)
+    tracer.launch(sc, plain)
+    R = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(ring_band=1.0, audit_every=5))
+    R.run()
+    torch.cuda.synchronize()
+    P = tracer.records_to_numpy(plain)
+    Sr = tracer.records_to_numpy(R.out)
+    fb, au = R.counts_host()
+    assert fb > 0 and au > 0
+    fb_ids = R.fb_ids[:fb].cpu().numpy()
+    a = R.out.cpu().numpy()[fb_ids]
+    b = plain.cpu().numpy()[fb_ids]
+    assert np.array_equal(a, b), "fallback records differ from the plain integrator's"
+    assert ((Sr["flags"][fb_ids] & abi.FLAG_SURROGATE) == 0).all()
+    au_ids = R.audit_ids[:au].cpu().numpy()
+    comp = R.audit_rec[:au].cpu().numpy()
+    assert np.array_equal(comp, plain.cpu().numpy()[au_ids])
+    st = R.audit_stats()
+    assert st["n"] == au
+    I_s, I_p = Sr["intensity"][au_ids], P["intensity"][au_ids]
+    m = I_p > 0
+    assert abs(st["sum_rel"] - np.sum(np.abs(I_s[m] - I_p[m]) / I_p[m])) <= 1e-9 * max(st["sum_rel"], 1.0)
+    assert st["class_diff"] == int((Sr["hit_class"][au_ids] != P["hit_class"][au_ids]).sum())
+
+
+def test_light_bending_feature_tracks_the_oracle_first_crossing():
+    """Feature 14 = ln(r_bel)/3, the light-bending estimate of the first
+    equatorial crossing for an observer at 100 M, against the oracle's traced
+    first crossing (rays crossing outside r = 4).  A flipped image-plane sign
+    or a missing observer-distance term raises the error above 0.1."""
+    import math
+    import torch
+    import oracle_lib as OL
+    from blackhole_amd import abi, analysis, tracer
+    from blackhole_amd import surrogate as S
+    for th, limit in ((20.0, 0.01), (60.0, 0.03)):
+        cam = tracer.camera(0.0, th, 100.0, 17, 17, 18.0)
+        opts = tracer.default_options(integrator=abi.INTEGRATOR_DP45, disk_enabled=0, max_crossings=3,
+                                      max_steps=20000)
+        rec = OL.trace([cam], opts)
+        A, B = analysis.pixel_grid(cam)
+        rays = np.stack([A.reshape(-1), B.reshape(-1), np.zeros(A.size), np.full(A.size, math.cos(cam.theta_obs))],
+                        1).astype(np.float32)
+        f = S.Surrogate.features(torch.from_numpy(rays).cuda()).cpu().numpy()
+        m = (rec["n_cross"] >= 1) & (rec["cross_r"][:, 0] > 4.0)
+        err = np.abs(3.0 * f[m, 14] - np.log(rec["cross_r"][m, 0]))
+        flat = np.abs(3.0 * f[m, 15] - np.log(rec["cross_r"][m, 0]))
+        print("theta", th, "light-bending", err.mean(), "flat", flat.mean(), flush=True)
+        assert err.mean() < limit and err.mean() < 0.2 * flat.mean()
