@@ -1,14 +1,30 @@
 #!/usr/bin/env python3
 """Throughput benchmark of the MI355X Kerr tracer (one JSON line on stdout).
 
-A "step" is one full frame of the c2 workload: 1024 x 1024 pixels, spin 0.94,
-inclination 17 deg, adaptive Dormand-Prince 5(4) (rtol 1e-8, atol 1e-10) with
-volumetric thermal-synchrotron transfer at 230 GHz through the seeded
-Fishbone-Moncrief x lognormal torus on a 256 x 128 x 64 grid, FP64.  Inputs
-(camera table, volume, K2 table) are resident in HBM before timing; the timed
-region is the trace kernel plus its queue reset, per frame.
+Default workload (the surrogate comparison, ``--workload c5``): 16 frames of
+2048 x 2048 pixels over alpha, beta in [-20, 20] M from r_obs = 100 M, spins
+{0, 0.5, 0.9, 0.998} x inclinations {17, 40, 60, 85} deg, thin Novikov-Thorne
+disk r_ISCO -> 20 M observed at 1 keV (T_max 1e7 K).  A step is one pass of a
+method over all 16 frames (67.1 M rays), inputs resident in HBM:
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+  plain        the plain GPU integrator: FP64 Dormand-Prince 5(4), rtol 1e-8,
+               atol 1e-10 (k_trace_fp64<DP45>); its images are the FP64
+               reference (bitwise equal to the oracle's exit states, checked
+               on a sample below);
+  surrogate    the fused BF16-MFMA surrogate over every pixel (k_sur_render),
+               the integrator over its fallback rays (photon-ring band, disk
+               edges, ambiguous outputs), and the 1 % audit traced and compared.
+
+Accuracy of the surrogate images against the plain ones: mean relative
+intensity error over pixels with I_ref > 1e-6 max(I_ref), photon-ring
+(shadow) diameter error, hit-class disagreements.  ``value`` is the rays/s of
+the fastest method that meets the gate (mean relative intensity error <=
+1e-3 and ring-diameter error <= 0.1 %); both methods' rates are reported.
+
+``--workload c2`` times the volumetric FP64 DP45 tracer on the c2 frame
+(1024^2, a = 0.94, 17 deg, torus 256 x 128 x 64, 230 GHz) instead.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c2]
 """
 from __future__ import annotations
 
@@ -23,77 +39,140 @@ sys.path.insert(0, ROOT)
 
 METRIC = "rays/s at ≤1e-3 rel intensity err: surrogate+fallback vs plain GPU DP45"
 FP64_VECTOR_PEAK_TFLOPS = 78.6
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+WEIGHTS = os.path.join(ROOT, "blackhole_amd", "weights", "surrogate_c5.npz")
 
 
-def cpu_baseline(w, gpu_rec, sample_stride: int):
-    """Time the FP64 CPU oracle (the checker) on a strided sample of the frame."""
-    import numpy as np
+def cpu_baseline(w, gpu_rec_np_fn, ids, label: str):
+    """Time the FP64 CPU oracle (the checker) on the listed pixels and check
+    the GPU records at those pixels against it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     from parity_util import compare, run_oracle
-    n = w["cams"][0].width * w["cams"][0].height
-    ids = np.arange(0, n, sample_stride, dtype=np.int64)
     threads = min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
     ref = run_oracle(oracle_lib, w, ids=ids)
     dt = time.perf_counter() - t0
-    par = compare(gpu_rec[ids], ref)
+    par = compare(gpu_rec_np_fn(ids), ref)
     return {
         "value": ids.size / dt,
         "unit": "rays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"every {sample_stride}th pixel of the same 1024x1024 c2 frame "
-                  f"({ids.size} rays, {dt:.1f} s wall on {threads} threads, FP64 oracle, -O3 -march=native)",
+        "sample": f"{label} ({ids.size} rays, {dt:.1f} s wall on {threads} threads, FP64 oracle, "
+                  f"-O3 -march=native)",
     }, par
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run one 512 MiB device copy (profiler counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "rays are independent and would shard by rows or cameras without exchange", file=sys.stderr)
-        return 2
+def time_steps(fn, steps: int, warmup: int):
+    """Warm up, then time exactly `steps` calls bracketed by synchronisations;
+    returns (wall ms per step, event ms per step on the current stream)."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1000.0 / steps, e0.elapsed_time(e1) / steps
 
+
+def frame_accuracy(sur_rec, plain_rec, cams):
+    """Per-frame and pooled accuracy of the surrogate records against the
+    plain ones, on the device (records are uint8 [n, 128] tensors)."""
+    import math
+    import torch
+    from blackhole_amd import abi
+    I_s = sur_rec.view(torch.float64)[:, 0]
+    I_p = plain_rec.view(torch.float64)[:, 0]
+    c_s = sur_rec[:, 112]
+    c_p = plain_rec[:, 112]
+    frames = []
+    tot_rel, tot_n, tot_abs, tot_ref = 0.0, 0, 0.0, 0.0
+    for cam in cams:
+        a, b = cam.record_offset, cam.record_offset + cam.width * cam.height
+        ip, is_ = I_p[a:b], I_s[a:b]
+        m = ip > 1e-6 * ip.max()
+        rel = (is_[m] - ip[m]).abs() / ip[m]
+        pix = (cam.alpha_max - cam.alpha_min) / cam.width * (cam.beta_max - cam.beta_min) / cam.height
+        d_s = 2.0 * math.sqrt(float((c_s[a:b] == abi.HIT_HORIZON).sum()) * pix / math.pi)
+        d_p = 2.0 * math.sqrt(float((c_p[a:b] == abi.HIT_HORIZON).sum()) * pix / math.pi)
+        f = {
+            "spin": cam.spin, "theta_deg": round(math.degrees(cam.theta_obs), 3),
+            "mean_rel": float(rel.mean()), "max_rel": float(rel.max()),
+            "frac_rel_gt_1e-2": float((rel > 1e-2).double().mean()),
+            "flux_weighted": float((is_ - ip).abs().sum() / ip.sum()),
+            "ring_diameter_rel_err": abs(d_s - d_p) / d_p,
+            "class_disagreements": int((c_s[a:b] != c_p[a:b]).sum()),
+        }
+        frames.append(f)
+        tot_rel += float(rel.sum())
+        tot_n += int(m.sum())
+        tot_abs += float((is_ - ip).abs().sum())
+        tot_ref += float(ip.sum())
+    return {
+        "mean_rel": tot_rel / max(tot_n, 1),
+        "flux_weighted": tot_abs / tot_ref,
+        "max_frame_mean_rel": max(f["mean_rel"] for f in frames),
+        "max_ring_diameter_rel_err": max(f["ring_diameter_rel_err"] for f in frames),
+        "class_disagreements": sum(f["class_disagreements"] for f in frames),
+        "frames": frames,
+    }
+
+
+def device_frame_flops(rec):
+    """flops.frame_flops of a DP45 thin-disk frame from the device records
+    (uint8 [n, 128]: steps and attempts are int32 words 26, 27; hit class
+    and crossing count bytes 112, 113)."""
+    import torch
+    from blackhole_amd import abi, flops
+    w = rec.view(torch.int32)
+    steps = float(w[:, 26].double().sum())
+    attempts = float(w[:, 27].double().sum())
+    cls = rec[:, 112]
+    n_valid = float((cls != 0).sum())
+    ncross = float(rec[:, 113].double().sum())
+    disk = float((cls == abi.HIT_DISK).sum())
+    f = (attempts * flops.DP45_ATTEMPT + steps * flops.PROJECTION + n_valid * flops.SETUP
+         + ncross * flops.CROSSING + disk * flops.DISK_HIT)
+    return f, attempts
+
+
+def bench_c5(args):
     import numpy as np
     import torch
+    from blackhole_amd import flops, sur_render, tracer, workloads
+    from blackhole_amd import surrogate_train as ST
 
-    from blackhole_amd import abi, flops, tracer, workloads
-
-    if not torch.cuda.is_available():
-        print("bench.py: no HIP device", file=sys.stderr)
-        return 1
-    w = workloads.c2(args.size, args.size)
-    if args.precision == "fp32":
-        w["opts"].precision = abi.PRECISION_FP32
-    sc = tracer.upload_scene(w["cams"], w["opts"], luts=w["luts"], cells=w["cells"], vol=w["vol"])
-    out = tracer.alloc_records(sc.n_records)
+    w = workloads.c5(args.size)
+    sc = tracer.upload_scene(w["cams"], w["opts"], luts=w["luts"])
+    n = sc.n_records
+    plain = tracer.alloc_records(n)
     stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        tracer.launch(sc, out, stream)
-    torch.cuda.synchronize()
 
-    kernel_ms = []
+    # ---- plain GPU DP45 -------------------------------------------------------
+    plain_ms, plain_ev_ms = time_steps(lambda: tracer.launch(sc, plain, stream), args.steps, args.warmup)
+    tracer.launch(sc, plain, stream)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tracer.launch(sc, out, stream)
-        torch.cuda.synchronize()
-        kernel_ms.append(tracer.last_kernel_ms())
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    rays = sc.n_records
-    value = rays / (ms_per_step / 1000.0)
+    plain_kernel_ms = tracer.last_kernel_ms()
+
+    # ---- surrogate + fallback + audit --------------------------------------------
+    sur = ST.load(WEIGHTS)
+    R = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig())
+    R.floor_I = 1e-6 * float(plain.view(torch.float64)[:, 0].max())
+    sur_ms, sur_ev_ms = time_steps(R.run, args.steps, args.warmup)
+    # the render kernel alone (roofline of the MFMA kernel)
+    _, render_ev_ms = time_steps(R.render, args.steps, 1)
+    R.run()
+    torch.cuda.synchronize()
+    fb, au = R.counts_host()
+    audit = R.audit_stats()
+    acc = frame_accuracy(R.out, plain, w["cams"])
 
     if args.copy_calibration:
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -102,39 +181,128 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
 
-    rec = tracer.records_to_numpy(out)
-    f = flops.frame_flops(rec, w["opts"].integrator)
-    k_ms = float(np.mean(kernel_ms))
-    achieved = f / (k_ms / 1000.0) / 1e12
+    plain_rate = n / (plain_ms / 1000.0)
+    sur_rate = n / (sur_ms / 1000.0)
+    gate = acc["mean_rel"] <= 1e-3 and acc["max_ring_diameter_rel_err"] <= 1e-3
+    use_sur = gate and sur_rate > plain_rate
+
+    # algorithmic FLOPs of the plain frame (counting oracle per step x executed attempts)
+    # and of the network (dense MACs x 2 per ray)
+    f_plain, attempts = device_frame_flops(plain)
+    dims = [(16, 256)] + [(256, 256)] * (sur.depth - 1) + [(256, 16)]
+    f_ray = 2.0 * sum(a * b for a, b in dims)
+    f_sur = f_ray * n
+    roof_plain = {
+        "bound": "valu", "achieved": f_plain / (plain_kernel_ms / 1000.0) / 1e12,
+        "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s", "traffic": None,
+        "kernel": "k_trace_fp64<DP45, no volume>", "kernel_ms": plain_kernel_ms, "flop_per_launch": f_plain,
+    }
+    roof_plain["frac"] = roof_plain["achieved"] / roof_plain["peak"]
+    roof_sur = {
+        "bound": "mfma", "achieved": f_sur / (render_ev_ms / 1000.0) / 1e12,
+        "peak": BF16_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s", "traffic": None,
+        "kernel": "k_sur_render (BF16 MFMA, FP32 accumulation)", "kernel_ms": render_ev_ms,
+        "flop_per_launch": f_sur, "flop_per_ray": f_ray,
+    }
+    roof_sur["frac"] = roof_sur["achieved"] / roof_sur["peak"]
+
     res = {
         "metric": METRIC,
-        "value": value,
+        "value": sur_rate if use_sur else plain_rate,
         "unit": "rays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": sur_ms if use_sur else plain_ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16" if use_sur else "f64",
+        "data": "synthetic (analytic cameras and disk; surrogate weights trained on 2^24 traced rays)",
+        "config": {
+            "workload": f"c5: 16 frames {args.size}x{args.size} (spin 0/0.5/0.9/0.998 x incl. 17/40/60/85 deg), "
+                        f"alpha,beta in [-20,20] M, r_obs 100 M, thin NT disk r_isco-20 M at 1 keV",
+            "rays": n,
+            "method_of_value": "surrogate+fallback" if use_sur else "plain GPU DP45 (surrogate misses the gate)",
+        },
+        "roofline": roof_sur if use_sur else roof_plain,
+        "plain_dp45": {
+            "rays_per_s": plain_rate, "ms_per_step": plain_ms, "event_ms_per_step": plain_ev_ms,
+            "attempts_mean": attempts / n, "roofline": roof_plain,
+        },
+        "surrogate_fallback": {
+            "rays_per_s": sur_rate, "ms_per_step": sur_ms, "event_ms_per_step": sur_ev_ms,
+            "render_kernel_ms": render_ev_ms, "speedup_vs_plain": sur_rate / plain_rate,
+            "fallback_rays": fb, "fallback_frac": fb / n, "audit_rays": au,
+            "gate_pass": gate, "accuracy_vs_plain_fp64": {k: v for k, v in acc.items() if k != "frames"},
+            "audit": {k: audit[k] for k in ("n", "mean_rel", "max_rel", "class_diff", "flux_weighted")},
+            "roofline": roof_sur,
+        },
+        "frames": acc["frames"],
+    }
+    if not args.no_cpu_baseline:
+        rng = np.random.default_rng(20261203)
+        ids = np.sort(rng.choice(n, size=min(args.cpu_sample, n), replace=False)).astype(np.int64)
+        ids_t = torch.from_numpy(ids).cuda()
+
+        def gpu_rec(sel):
+            return tracer.records_to_numpy(plain[ids_t])
+
+        cb, par = cpu_baseline(w, gpu_rec, ids,
+                               f"{ids.size} seeded random pixels of the 16 c5 frames, plain DP45 FP64")
+        res["cpu_baseline"] = cb
+        res["plain_gpu_vs_cpu"] = plain_rate / cb["value"]
+        res["parity_vs_oracle_on_sample"] = par
+    return res
+
+
+def bench_c2(args):
+    import numpy as np
+    import torch
+    from blackhole_amd import abi, flops, tracer, workloads
+
+    size = args.size if args.size != 2048 else 1024
+    w = workloads.c2(size, size)
+    if args.precision == "fp32":
+        w["opts"].precision = abi.PRECISION_FP32
+    sc = tracer.upload_scene(w["cams"], w["opts"], luts=w["luts"], cells=w["cells"], vol=w["vol"])
+    out = tracer.alloc_records(sc.n_records)
+    stream = torch.cuda.current_stream()
+    ms, _ = time_steps(lambda: tracer.launch(sc, out, stream), args.steps, args.warmup)
+    tracer.launch(sc, out, stream)
+    torch.cuda.synchronize()
+    k_ms = tracer.last_kernel_ms()
+    if args.copy_calibration:
+        src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    rec = tracer.records_to_numpy(out)
+    f = flops.frame_flops(rec, w["opts"].integrator)
+    achieved = f / (k_ms / 1000.0) / 1e12
+    res = {
+        "metric": METRIC,
+        "value": sc.n_records / (ms / 1000.0),
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic (seeded torus volume, analytic camera)",
         "config": {
-            "workload": f"c2: plain GPU DP45 FP64 + volumetric synchrotron, a=0.94, 17 deg, "
-                        f"{args.size}x{args.size}, rtol 1e-8 atol 1e-10, torus 256x128x64",
-            "form": "plain DP45 tracer (the surrogate+fallback form is not built yet)",
-            "rays": rays,
+            "workload": f"c2: plain GPU DP45 {args.precision.upper()} + volumetric synchrotron, a=0.94, 17 deg, "
+                        f"{size}x{size}, rtol 1e-8 atol 1e-10, torus 256x128x64, 230 GHz",
+            "rays": sc.n_records,
         },
         "roofline": {
-            "bound": "valu",
-            "achieved": achieved,
-            "peak": FP64_VECTOR_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel": "k_trace_fp64<DP45, volume>",
-            "kernel_ms": k_ms,
-            "flop_per_frame": f,
+            "bound": "valu", "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": None,
+            "kernel": f"k_trace_{args.precision}<DP45, volume>", "kernel_ms": k_ms, "flop_per_launch": f,
         },
         "frame_stats": {
             "attempts_mean": float(rec["attempts"].mean()),
@@ -146,9 +314,38 @@ def main() -> int:
         },
     }
     if not args.no_cpu_baseline:
-        cb, par = cpu_baseline(w, rec, args.cpu_stride)
+        n = sc.n_records
+        ids = np.arange(0, n, args.cpu_stride, dtype=np.int64)
+        cb, par = cpu_baseline(w, lambda sel: rec[sel], ids,
+                               f"every {args.cpu_stride}th pixel of the same {size}x{size} c2 frame")
         res["cpu_baseline"] = cb
         res["parity_vs_oracle_on_sample"] = par
+    return res
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["c5", "c2"], default="c5")
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=8_000_000, help="c5: oracle pixels for the CPU baseline")
+    ap.add_argument("--cpu-stride", type=int, default=1, help="c2: oracle pixel stride")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64", help="c2 only")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 512 MiB device copy (profiler counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "rays are independent and would shard by rows or cameras without exchange", file=sys.stderr)
+        return 2
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+    res = bench_c5(args) if args.workload == "c5" else bench_c2(args)
     print(json.dumps(res), flush=True)
     return 0
 
