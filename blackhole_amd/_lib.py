@@ -61,6 +61,22 @@ def lib():
     L.bh_trace_pixel_list.restype = C.c_int
     L.bh_trace_pixel_list_compact.argtypes = L.bh_trace_pixel_list.argtypes
     L.bh_trace_pixel_list_compact.restype = C.c_int
+    L.bh_trace_image_plane_multi.argtypes = [
+        C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions), C.c_void_p, C.c_int,
+        C.POINTER(abi.BH_VolumeSet), C.c_void_p, C.c_void_p]
+    L.bh_trace_image_plane_multi.restype = C.c_int
+    L.bh_frame_create.argtypes = [
+        C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions), C.c_void_p, C.c_int,
+        C.POINTER(abi.BH_Volume), C.POINTER(abi.BH_VolumeSet), C.c_void_p]
+    L.bh_frame_create.restype = C.c_void_p
+    L.bh_frame_launch.argtypes = [C.c_void_p, C.c_void_p]
+    L.bh_frame_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
+    L.bh_frame_destroy.argtypes = [C.c_void_p]
+    L.bh_frame_destroy.restype = None
+    L.bh_frame_set_launch_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.bh_frame_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.BH_TuneResult), C.c_void_p]
+    for name in ("bh_frame_launch", "bh_frame_graph_launch", "bh_frame_set_launch_shape", "bh_frame_autotune"):
+        getattr(L, name).restype = C.c_int
     L.bh_trace_rays.argtypes = [C.c_void_p, C.c_int64, C.POINTER(abi.BH_TraceOptions),
                                 C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p]
     L.bh_trace_rays.restype = C.c_int

@@ -66,6 +66,8 @@ class BH_TraceOptions(C.Structure):
         ("transmit_min", C.c_double),
         ("island_r_factor", C.c_double),
         ("island_turn_dist", C.c_double),
+        ("blocks_per_cu", C.c_int32),
+        ("pad_launch", C.c_int32),
     ]
 
 
@@ -82,6 +84,26 @@ class BH_Volume(C.Structure):
         ("sup_r_hi", C.c_double),
         ("sup_u_max", C.c_double),
         ("ell", C.c_double),
+    ]
+
+
+class BH_VolumeSet(C.Structure):
+    _fields_ = [
+        ("vols", C.POINTER(BH_Volume)),
+        ("n_vols", C.c_int32),
+        ("pad0", C.c_int32),
+        ("cam_vol", C.POINTER(C.c_int32)),
+    ]
+
+
+class BH_TuneResult(C.Structure):
+    _fields_ = [
+        ("blocks_per_cu", C.c_int32),
+        ("refill_batch", C.c_int32),
+        ("occupancy_limit", C.c_int32),
+        ("sample_rays", C.c_int32),
+        ("sample_ms", C.c_float),
+        ("pad0", C.c_int32),
     ]
 
 

@@ -73,7 +73,9 @@ __device__ __forceinline__ RayCT<T> convert_const(const RayC& c) {
 struct CamDev {
   double a, sin_o, cos_o, r_obs, r_stop, r_isco, r_plus;
   double alpha_min, dalpha, beta_max, dbeta;
-  int64_t w, h, offset, lut_index;
+  int64_t w, h, offset;
+  int32_t lut_index;  // disk LUT row
+  int32_t vol_index;  // volume of a volume set (multi-volume launches)
 };
 
 __device__ __forceinline__ RayC make_const(double a, double xi, double eta) {

@@ -13,6 +13,13 @@ namespace bhk {
 
 constexpr int kBlock = 256;
 
+// Per-volume data of a volume set (volumes sharing one grid geometry and
+// support): the cells and the fluid's angular momentum.
+struct VolBucket {
+  const float4* cells;
+  double ell;
+};
+
 // Launch parameters, resident in device memory next to the camera table
 // (one upload per launch).  The kernels read them through a pointer rather
 // than as ~350 bytes of kernel arguments, which kept ~100 scalar registers
@@ -28,6 +35,7 @@ struct DevParams {
   long long n_total;
   int n_cams, lut_n, batch, max_steps, max_attempts, max_crossings, disk_enabled;
   int compact_ids;  // id list: record of entry q -> out[q] instead of out[id]
+  const VolBucket* buckets;  // volume set (camera k samples buckets[k.vol_index]); NULL: P.vol
   // Ray sources other than every pixel of the cameras: an explicit ray list
   // (rays, n_total entries, record i -> out[i]) or a list of pixel ids
   // (ids, count read from count_dev, record -> out[id]).
@@ -75,6 +83,7 @@ __device__ __forceinline__ bool resolve_ray(const DevParams& P, long long q, lon
     k->r_stop = P.horizon_factor * rp;
     k->r_isco = 3.0 + z2 - __builtin_sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
     k->lut_index = 0;
+    k->vol_index = 0;
     *alpha = sp.alpha;
     *beta = sp.beta;
     *rec = q;
@@ -132,8 +141,9 @@ __device__ __forceinline__ long long refill_id(bool need, unsigned long long* he
 int make_cam_devs(const BH_Camera* cams, int n_cams, double horizon_factor, const char* who,
                   std::vector<CamDev>* out, long long* total);
 
-// FP32 precision form launcher (trace_fp32.hip).
-hipError_t launch_fp32(int integrator, bool vol_on, const DevParams* d_params, hipStream_t s,
-                       int grid_override);
+// FP32 precision form (trace_fp32.hip): launch on `grid` workgroups, and
+// the workgroups per CU its occupancy allows.
+hipError_t launch_fp32(int integrator, bool vol_on, const DevParams* d_params, hipStream_t s, int grid);
+int occupancy_limit_fp32(int integrator, bool vol_on);
 
 }  // namespace bhk

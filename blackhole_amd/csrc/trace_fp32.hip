@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_fp32(const DevParams* __restri
 }
 
 template <int I, bool V>
-int occupancy_grid_f() {
+int max_blocks_f() {
   static int cached[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
@@ -490,15 +490,12 @@ int occupancy_grid_f() {
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_fp32<I, V>, kBlock, 0) != hipSuccess)
     return -1;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return -1;
-  cached[dev] = (nb > 0 ? nb : 1) * prop.multiProcessorCount;
+  cached[dev] = nb > 0 ? nb : 1;
   return cached[dev];
 }
 
 template <int I, bool V>
-hipError_t launch_f(const DevParams* d_params, hipStream_t s, int grid_override) {
-  const int grid = grid_override > 0 ? grid_override : occupancy_grid_f<I, V>();
+hipError_t launch_f(const DevParams* d_params, hipStream_t s, int grid) {
   if (grid <= 0) return hipErrorInvalidConfiguration;
   hipLaunchKernelGGL((k_trace_fp32<I, V>), dim3(grid), dim3(kBlock), 0, s, d_params);
   return hipGetLastError();
@@ -506,14 +503,20 @@ hipError_t launch_f(const DevParams* d_params, hipStream_t s, int grid_override)
 
 }  // namespace f32
 
-// Entry used by bh_trace_image_plane for BH_PRECISION_FP32.
-hipError_t launch_fp32(int integrator, bool vol_on, const DevParams* d_params, hipStream_t s,
-                       int grid_override) {
+// Launch of the FP32 form on a grid of `grid` workgroups.
+hipError_t launch_fp32(int integrator, bool vol_on, const DevParams* d_params, hipStream_t s, int grid) {
   if (integrator == BH_INTEGRATOR_DP45)
-    return vol_on ? f32::launch_f<BH_INTEGRATOR_DP45, true>(d_params, s, grid_override)
-                  : f32::launch_f<BH_INTEGRATOR_DP45, false>(d_params, s, grid_override);
-  return vol_on ? f32::launch_f<BH_INTEGRATOR_RK4, true>(d_params, s, grid_override)
-                : f32::launch_f<BH_INTEGRATOR_RK4, false>(d_params, s, grid_override);
+    return vol_on ? f32::launch_f<BH_INTEGRATOR_DP45, true>(d_params, s, grid)
+                  : f32::launch_f<BH_INTEGRATOR_DP45, false>(d_params, s, grid);
+  return vol_on ? f32::launch_f<BH_INTEGRATOR_RK4, true>(d_params, s, grid)
+                : f32::launch_f<BH_INTEGRATOR_RK4, false>(d_params, s, grid);
+}
+
+// Workgroups of the FP32 kernel that fit on one CU.
+int occupancy_limit_fp32(int integrator, bool vol_on) {
+  if (integrator == BH_INTEGRATOR_DP45)
+    return vol_on ? f32::max_blocks_f<BH_INTEGRATOR_DP45, true>() : f32::max_blocks_f<BH_INTEGRATOR_DP45, false>();
+  return vol_on ? f32::max_blocks_f<BH_INTEGRATOR_RK4, true>() : f32::max_blocks_f<BH_INTEGRATOR_RK4, false>();
 }
 
 }  // namespace bhk

@@ -189,3 +189,75 @@ def records_to_numpy(out) -> np.ndarray:
 
 def last_kernel_ms() -> float:
     return float(lib().bh_last_kernel_ms())
+
+
+class VolumeSetDevice:
+    """Device cells of several volumes sharing one geometry (a volume set);
+    keeps the tensors and the ctypes BH_VolumeSet alive."""
+
+    def __init__(self, cells_list, vols, cam_vol, device: str = "cuda:0"):
+        torch = _torch()
+        self.cells = [torch.from_numpy(np.ascontiguousarray(c)).to(device) for c in cells_list]
+        self.vols = (abi.BH_Volume * len(vols))()
+        for i, v in enumerate(vols):
+            C.pointer(self.vols[i])[0] = v
+            self.vols[i].cells = self.cells[i].data_ptr()
+        self.cam_vol = (C.c_int32 * len(cam_vol))(*[int(x) for x in cam_vol])
+        self.vs = abi.BH_VolumeSet(self.vols, len(vols), 0, self.cam_vol)
+
+
+def launch_multi(sc: DeviceScene, vset: VolumeSetDevice, out, stream=None) -> None:
+    """Asynchronous trace of every pixel of sc.cams, camera i sampling volume
+    vset.cam_vol[i] (bh_trace_image_plane_multi)."""
+    torch = _torch()
+    cam_arr = (abi.BH_Camera * len(sc.cams))(*sc.cams)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib().bh_trace_image_plane_multi(
+        cam_arr, len(sc.cams), C.byref(sc.opts),
+        C.c_void_p(sc.luts.data_ptr()) if sc.luts is not None else None, sc.lut_n,
+        C.byref(vset.vs), C.c_void_p(out.data_ptr()), C.c_void_p(s.cuda_stream)), "bh_trace_image_plane_multi")
+
+
+class Frame:
+    """A prepared frame (bh_frame_*): launches without host work, optionally
+    replayed from a captured hipGraph, with an autotuned launch shape."""
+
+    def __init__(self, sc: DeviceScene, out, vset: Optional[VolumeSetDevice] = None):
+        self.sc, self.out, self.vset = sc, out, vset
+        cam_arr = (abi.BH_Camera * len(sc.cams))(*sc.cams)
+        self.h = lib().bh_frame_create(
+            cam_arr, len(sc.cams), C.byref(sc.opts),
+            C.c_void_p(sc.luts.data_ptr()) if sc.luts is not None else None, sc.lut_n,
+            C.byref(sc.vol) if (sc.vol is not None and vset is None) else None,
+            C.byref(vset.vs) if vset is not None else None, C.c_void_p(out.data_ptr()))
+        if not self.h:
+            check(-1, "bh_frame_create")
+
+    def _s(self, stream):
+        torch = _torch()
+        return C.c_void_p((stream if stream is not None else torch.cuda.current_stream()).cuda_stream)
+
+    def launch(self, stream=None) -> None:
+        check(lib().bh_frame_launch(self.h, self._s(stream)), "bh_frame_launch")
+
+    def graph_launch(self, stream=None) -> None:
+        check(lib().bh_frame_graph_launch(self.h, self._s(stream)), "bh_frame_graph_launch")
+
+    def set_launch_shape(self, blocks_per_cu: int, refill_batch: int) -> None:
+        check(lib().bh_frame_set_launch_shape(self.h, blocks_per_cu, refill_batch), "bh_frame_set_launch_shape")
+
+    def autotune(self, sample_stride: int = 8, stream=None) -> abi.BH_TuneResult:
+        r = abi.BH_TuneResult()
+        check(lib().bh_frame_autotune(self.h, sample_stride, C.byref(r), self._s(stream)), "bh_frame_autotune")
+        return r
+
+    def close(self) -> None:
+        if self.h:
+            lib().bh_frame_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -10,7 +10,11 @@ c2  DP45 (rtol 1e-8, atol 1e-10) plus volumetric thermal synchrotron at
     a = 0.94, 17 deg, 1024^2 over alpha, beta in [-20, 20]; FP64.
 c3  precision form: a = 0.998, 85 deg, 4096^2, disk + the c2 torus recipe.
 c4  batched: 256 cameras (N ~ logU(64, 1024), a ~ U[0, 0.998],
-    theta ~ U[5, 85] deg, r_obs ~ U[50, 1000]) from a fixed seed.
+    theta ~ U[5, 85] deg, r_obs ~ U[50, 1000]) from a fixed seed; thin disk
+    plus the c2 torus recipe on 128 x 64 x 32 per spin bucket (8 buckets of
+    width 0.998/8, torus at the bucket's mid spin), all buckets on one grid
+    (r from the horizon of a = 0.998 to 50 M) and one support; DP45 FP64
+    at the c2 tolerances, 230 GHz.
 c5  surrogate evaluation frames: 2048^2 over alpha, beta in [-20, 20] from
     r_obs = 100 on a 4 x 4 sweep a in {0, 0.5, 0.9, 0.998} x theta in
     {17, 40, 60, 85} deg; the c1 disk (NT, r_isco -> 20 M, 1 keV, T_max
@@ -23,7 +27,7 @@ import math
 import numpy as np
 
 from . import abi
-from .tracer import camera, default_options, disk_luts, generate_torus, pack_cameras, torus_params
+from .tracer import camera, default_options, disk_luts, generate_torus, outer_horizon, pack_cameras, torus_params
 
 KEV_HZ = 2.417989242e17
 
@@ -95,3 +99,33 @@ def c5(size: int = 2048, spins=C5_SPINS, inclinations=C5_INCLINATIONS):
     cams = [camera(a, th, 100.0, size, size, 20.0) for a in spins for th in inclinations]
     pack_cameras(cams)
     return dict(cams=cams, opts=c5_options(), luts=disk_luts(cams, 20.0), cells=None, vol=None)
+
+
+C4_BUCKETS = 8
+
+
+def c4(n: int = 256, seed: int = 0x5EED0004, grid=(128, 64, 32), buckets: int = C4_BUCKETS):
+    """The batched form: cameras, per-spin-bucket tori sharing one geometry,
+    the camera -> bucket map, disk LUTs and options."""
+    cams = c4_cameras(n, seed)
+    width = 0.998 / buckets
+    cam_vol = [min(int(c.spin / width), buckets - 1) for c in cams]
+    r_lo = outer_horizon(0.998)
+    cells, vols = [], []
+    for b in range(buckets):
+        tp = torus_params((b + 0.5) * width, n_r=grid[0], n_th=grid[1], n_ph=grid[2], seed=seed + 17 * b)
+        tp.r_lo = r_lo
+        c, v = generate_torus(tp)
+        cells.append(c)
+        vols.append(v)
+    lo = min(v.sup_r_lo for v in vols)
+    hi = max(v.sup_r_hi for v in vols)
+    um = max(v.sup_u_max for v in vols)
+    for v in vols:
+        v.sup_r_lo, v.sup_r_hi, v.sup_u_max = lo, hi, um
+    opts = default_options(
+        integrator=abi.INTEGRATOR_DP45, precision=abi.PRECISION_FP64, max_steps=20000,
+        max_attempts=100000, disk_enabled=1, volume_enabled=1, rtol=1e-8, atol=1e-10,
+        h_init=0.01, h_max=0.1, h_vol=0.01, horizon_factor=1.01, r_escape=200.0,
+        disk_r_out=20.0, nu_obs=230.0e9, disk_T_max=1.0e7)
+    return dict(cams=cams, opts=opts, luts=disk_luts(cams, 20.0), cells_list=cells, vols=vols, cam_vol=cam_vol)

@@ -129,6 +129,10 @@ typedef struct BH_TraceOptions {
     double  island_r_factor;/* FP32 form: FP64 steps when r < this * r_+        */
     double  island_turn_dist;/* FP32 form: FP64 steps within this [M] of a
                                radial or polar turning point                    */
+    int32_t blocks_per_cu;  /* persistent-kernel workgroups per CU (0 = the
+                               autotuned value for this device and kernel if
+                               one is cached, else the occupancy limit)         */
+    int32_t pad_launch;
 } BH_TraceOptions;
 
 /* Synthetic GRMHD-like volume on a Boyer-Lindquist (r, theta, phi) grid.
@@ -262,6 +266,63 @@ int bh_trace_pixel_list_compact(const BH_Camera* cams, int n_cameras,
  * the volume may be enabled. */
 int bh_trace_rays(const BH_RaySpec* rays, int64_t n, const BH_TraceOptions* opts,
                   const BH_Volume* vol, BH_PixelRecord* out, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Batched frames: volume sets, prepared launches, graphs, autotuning     */
+/* ---------------------------------------------------------------------- */
+
+/* Several volumes sharing one grid geometry and support (n_r, n_th, n_ph,
+ * r_min, r_max, sup_*), differing in their cells and fluid angular momentum
+ * ell -- e.g. one torus per spin bucket of a camera batch.  Camera i samples
+ * vols[cam_vol[i]] (host arrays; cells are device pointers). */
+typedef struct BH_VolumeSet {
+    const BH_Volume* vols;
+    int32_t n_vols;
+    int32_t pad0;
+    const int32_t* cam_vol;   /* n_cameras entries in [0, n_vols) */
+} BH_VolumeSet;
+
+/* bh_trace_image_plane with a volume set (FP64 form; volume_enabled set):
+ * every camera of the batch in one persistent launch. */
+int bh_trace_image_plane_multi(const BH_Camera* cams, int n_cameras,
+                               const BH_TraceOptions* opts,
+                               const double* disk_luts, int disk_lut_n,
+                               const BH_VolumeSet* vs,
+                               BH_PixelRecord* out, void* stream);
+
+/* A prepared frame: cameras, options, tables and the output buffer validated
+ * once and kept in device memory, so that launching the frame needs no host
+ * work.  bh_frame_launch enqueues the queue reset and the persistent kernel
+ * (capturable); bh_frame_graph_launch captures that launch into a hipGraph on
+ * its first call (on a stream of the frame's own) and replays the graph on
+ * `stream` afterwards.  vol or vs (or neither) as in the one-shot calls.
+ * NULL on failure (bh_last_error).  Device pointers must outlive the frame. */
+typedef struct BH_Frame BH_Frame;
+BH_Frame* bh_frame_create(const BH_Camera* cams, int n_cameras, const BH_TraceOptions* opts,
+                          const double* disk_luts, int disk_lut_n,
+                          const BH_Volume* vol, const BH_VolumeSet* vs,
+                          BH_PixelRecord* out);
+int  bh_frame_launch(BH_Frame* f, void* stream);
+int  bh_frame_graph_launch(BH_Frame* f, void* stream);
+void bh_frame_destroy(BH_Frame* f);
+
+/* Launch shape of the persistent kernel: workgroups per CU (0 = occupancy
+ * limit) and steps between lane refills (0 = keep).  Drops a captured graph. */
+int bh_frame_set_launch_shape(BH_Frame* f, int blocks_per_cu, int refill_batch);
+
+/* Autotuner: times the frame's kernel on every sample_stride-th pixel (into a
+ * buffer of its own) for each workgroups-per-CU value up to the occupancy
+ * limit and refill batches {1, 2, 4, 8, 16, 32}; applies the fastest shape to
+ * the frame and caches it for this device and kernel (used by later launches
+ * whose options leave blocks_per_cu / refill_batch at 0).  Synchronises. */
+typedef struct BH_TuneResult {
+    int32_t blocks_per_cu, refill_batch;
+    int32_t occupancy_limit;    /* workgroups per CU the kernel's registers allow */
+    int32_t sample_rays;
+    float   sample_ms;          /* best trial time on the sample                  */
+    int32_t pad0;
+} BH_TuneResult;
+int bh_frame_autotune(BH_Frame* f, int sample_stride, BH_TuneResult* best, void* stream);
 
 /* Elapsed time of the last bh_trace_image_plane kernel on this thread, from
  * HIP events recorded around the launch on its stream (ms; -1 if none).  The

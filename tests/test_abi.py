@@ -104,3 +104,63 @@ def test_null_arguments_rejected():
     o.volume_enabled = 1
     assert lib().bh_trace_image_plane(cams, 1, C.byref(o), C.c_void_p(1), 256, None,
                                       C.c_void_p(1), None) == -1
+
+
+def test_struct_layouts_match_the_c_headers(tmp_path):
+    """sizeof / offsetof of every ABI struct from a C compile of include/*.h
+    against the ctypes mirrors."""
+    import ctypes as C
+    import subprocess
+    from blackhole_amd import abi
+    from blackhole_amd import surrogate as S
+    structs = {
+        "BH_Camera": abi.BH_Camera, "BH_TraceOptions": abi.BH_TraceOptions, "BH_Volume": abi.BH_Volume,
+        "BH_VolumeSet": abi.BH_VolumeSet, "BH_TuneResult": abi.BH_TuneResult, "BH_RaySpec": abi.BH_RaySpec,
+        "BH_SurrogateRenderParams": S.RenderParams, "BH_SurrogateConfig": S.SurrogateConfig,
+        "BH_TorusParams": abi.BH_TorusParams,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bhk_trace.h"', '#include "bhk_surrogate.h"',
+             "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append(f'  printf("{name} sizeof %zu\\n", sizeof({name}));')
+        for f in cls._fields_:
+            lines.append(f'  printf("{name} {f[0]} %zu\\n", offsetof({name}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in got:
+        if not line:
+            continue
+        name, field, val = line.split()
+        cls = structs[name]
+        want = C.sizeof(cls) if field == "sizeof" else getattr(cls, field).offset
+        assert int(val) == want, (name, field, int(val), want)
+
+
+def test_frame_and_volume_set_validation_without_gpu():
+    import ctypes as C
+    from blackhole_amd import abi, tracer
+    from blackhole_amd._lib import lib
+    cams = (abi.BH_Camera * 1)(tracer.camera(1.2, 60.0, 100.0, 8, 8))  # spin out of range
+    opts = tracer.default_options(disk_enabled=0)
+    assert not lib().bh_frame_create(cams, 1, C.byref(opts), None, 0, None, None, C.c_void_p(1))
+    assert b"spin" in lib().bh_last_error()
+    cams = (abi.BH_Camera * 2)(tracer.camera(0.5, 60.0, 100.0, 8, 8), tracer.camera(0.9, 30.0, 100.0, 8, 8))
+    tracer.pack_cameras(cams)
+    opts = tracer.default_options(volume_enabled=1, disk_enabled=0)
+    v0 = abi.BH_Volume(C.c_void_p(16), 8, 4, 2, 0, 1.1, 50.0, 5.0, 40.0, 0.7, 4.0)
+    v1 = abi.BH_Volume(C.c_void_p(32), 8, 4, 4, 0, 1.1, 50.0, 5.0, 40.0, 0.7, 4.0)  # n_ph differs
+    vols = (abi.BH_Volume * 2)(v0, v1)
+    cv = (C.c_int32 * 2)(0, 1)
+    vs = abi.BH_VolumeSet(vols, 2, 0, cv)
+    assert lib().bh_trace_image_plane_multi(cams, 2, C.byref(opts), None, 0, C.byref(vs), C.c_void_p(1), None) == -1
+    assert b"share grid geometry" in lib().bh_last_error()
+    cv2 = (C.c_int32 * 2)(0, 5)
+    vols2 = (abi.BH_Volume * 2)(v0, v0)
+    vs2 = abi.BH_VolumeSet(vols2, 2, 0, cv2)
+    assert lib().bh_trace_image_plane_multi(cams, 2, C.byref(opts), None, 0, C.byref(vs2), C.c_void_p(1), None) == -1
+    assert b"volume index" in lib().bh_last_error()
