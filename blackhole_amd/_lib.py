@@ -77,6 +77,18 @@ def lib():
     L.bh_frame_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.BH_TuneResult), C.c_void_p]
     for name in ("bh_frame_launch", "bh_frame_graph_launch", "bh_frame_set_launch_shape", "bh_frame_autotune"):
         getattr(L, name).restype = C.c_int
+    L.bh_launch_geodesic_kernel.argtypes = [C.c_void_p, C.POINTER(abi.BH_LaunchParams), C.c_int, C.c_void_p]
+    L.bh_launch_geodesic_kernel.restype = C.c_int
+    L.bh_trace_launch_params.argtypes = [C.POINTER(abi.BH_LaunchParams), C.c_int, C.c_void_p, C.c_void_p]
+    L.bh_trace_launch_params.restype = C.c_int
+    L.bh_select_kernel_variant.argtypes = []
+    L.bh_select_kernel_variant.restype = C.c_int
+    L.bh_upload_lut_textures.argtypes = [C.c_ulonglong] * 8
+    L.bh_upload_lut_textures.restype = None
+    L.bh_upload_bridge_background_texture.argtypes = [C.c_ulonglong]
+    L.bh_upload_bridge_background_texture.restype = None
+    L.bh_lut_texture_handle.argtypes = [C.c_int]
+    L.bh_lut_texture_handle.restype = C.c_ulonglong
     L.bh_trace_rays.argtypes = [C.c_void_p, C.c_int64, C.POINTER(abi.BH_TraceOptions),
                                 C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p]
     L.bh_trace_rays.restype = C.c_int

@@ -158,6 +158,71 @@ __device__ __forceinline__ RayC setup_ray(const CamDev& k, int64_t px, int64_t p
   return setup_ray_ab(k, alpha, beta, y);
 }
 
+// Pinhole camera at a finite position (the render-launch camera): position
+// in Boyer-Lindquist coordinates and the flat spherical basis there (host
+// values, so the device and the oracle start from identical bits), the
+// column-major camera basis [right, up, forward], and the pixel mapping.
+struct PinholeDev {
+  double r, cos_t, sin_t, phi;
+  double e_r[3], e_t[3], e_p[3];
+  double basis[9];
+  double fov, aspect, shift_x, shift_y, inv_w, inv_h;
+};
+
+// Constants and initial state of the ray through pixel centre (xc, yc) =
+// (px + 0.5, py + 0.5) of a pinhole camera: the pixel direction as
+// d_ray_dir maps it (src/cuda/device_physics.cuh:1953-1967), its projections
+// on the flat spherical basis taken as the coordinate components k^r,
+// k^theta, k^phi, k^t from the null condition of the Kerr metric (M = 1),
+// E = -(g_tt k^t + g_tphi k^phi), L = g_tphi k^t + g_phiphi k^phi and
+// p_theta = Sigma k^theta (d_kerr_init_consts, device_physics.cuh:316-392),
+// normalised to E = 1: xi = L/E, eta = (p_theta/E)^2 - a^2 cos^2 + xi^2 cot^2.
+// The Mino velocities take their magnitudes from the potentials, sqrt(R) and
+// sqrt(U), and their signs from k^r and -k^theta.  The camera must be outside
+// r = 2 (outside every ergosphere), so k^t is the root (hb + sqrt(disc)) / -g_tt.
+__device__ __forceinline__ RayC setup_pinhole(const PinholeDev& p, double a, double xc, double yc, double* y) {
+  double u = (2.0 * xc * p.inv_w - 1.0) * p.fov;
+  u = u * p.aspect + p.shift_x;
+  const double v = (2.0 * yc * p.inv_h - 1.0) * p.fov + p.shift_y;
+  const double lx = -u, ly = -v, lz = 1.0;
+  const double inv_n = 1.0 / __builtin_sqrt(lx * lx + ly * ly + lz * lz);
+  double d[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) d[i] = (p.basis[i] * lx + p.basis[3 + i] * ly + p.basis[6 + i] * lz) * inv_n;
+  const double r = p.r;
+  const double kr = d[0] * p.e_r[0] + d[1] * p.e_r[1] + d[2] * p.e_r[2];
+  const double kth = (d[0] * p.e_t[0] + d[1] * p.e_t[1] + d[2] * p.e_t[2]) / r;
+  const double kph = (d[0] * p.e_p[0] + d[1] * p.e_p[1] + d[2] * p.e_p[2]) / (r * p.sin_t);
+  const double cos2 = p.cos_t * p.cos_t, sin2 = p.sin_t * p.sin_t, a2 = a * a;
+  const double sigma = r * r + a2 * cos2;
+  const double delta = r * r - 2.0 * r + a2;
+  const double f = 2.0 * r / sigma;
+  const double g_tt = -(1.0 - f);
+  const double g_tp = -f * a * sin2;
+  const double g_rr = sigma / delta;
+  const double g_pp = (r * r + a2 + f * a2 * sin2) * sin2;
+  const double spatial = g_rr * kr * kr + sigma * kth * kth + g_pp * kph * kph;
+  const double hb = g_tp * kph;
+  const double disc = hb * hb - g_tt * spatial;
+  const double kt = (hb + __builtin_sqrt(disc)) / (-g_tt);
+  const double E = -(g_tt * kt + g_tp * kph);
+  const double L = g_tp * kt + g_pp * kph;
+  const double xi = L / E;
+  const double pth = sigma * kth / E;
+  const double eta = pth * pth - a2 * cos2 + xi * xi * cos2 / sin2;
+  const RayC c = make_const(a, xi, eta);
+  y[0] = r;
+  y[1] = p.cos_t;
+  y[2] = p.phi;
+  double R = radial_potential(c, r);
+  double U = polar_potential(c, p.cos_t);
+  if (R < 0.0) R = 0.0;
+  if (U < 0.0) U = 0.0;
+  y[3] = kr >= 0.0 ? __builtin_sqrt(R) : -__builtin_sqrt(R);
+  y[4] = kth >= 0.0 ? -__builtin_sqrt(U) : __builtin_sqrt(U);
+  return c;
+}
+
 // Classic RK4 step from y with k1 = f(y).
 template <class T>
 __device__ __forceinline__ void rk4_step(const RayCT<T>& c, const T* y, const T* k1, T h, T* yn) {

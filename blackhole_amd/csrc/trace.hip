@@ -55,7 +55,8 @@ __device__ __forceinline__ bool lane_setup(Lane& L, const DevParams& P, long lon
   CamDev k;
   double alpha, beta;
   if (!resolve_ray(P, q, &L.gid, &L.cam, &k, &alpha, &beta)) return false;
-  L.c = setup_ray_ab(k, alpha, beta, L.y);
+  // a pinhole camera's table entry maps pixels to their centres (px + 0.5, py + 0.5)
+  L.c = P.pin ? setup_pinhole(*P.pin, k.a, alpha, beta, L.y) : setup_ray_ab(k, alpha, beta, L.y);
   rhs(L.c, L.y, L.k1);
   L.h = P.h_init;
   L.rt.I = 0.0;
@@ -382,11 +383,13 @@ struct LaunchImage {
   DevParams P;
   std::vector<CamDev> cams;
   std::vector<VolBucket> buckets;
+  std::vector<PinholeDev> pin;  // 0 or 1 entries
   KernelKey key;
   int blocks_per_cu;  // 0: tuned or occupancy limit
 
   size_t bytes() const {
-    return sizeof(DevParams) + cams.size() * sizeof(CamDev) + buckets.size() * sizeof(VolBucket);
+    return sizeof(DevParams) + cams.size() * sizeof(CamDev) + buckets.size() * sizeof(VolBucket) +
+           pin.size() * sizeof(PinholeDev);
   }
   // Serialises into host memory dst, with the table pointers of the block
   // fixed up for its device address d_block and the ray-queue head the
@@ -398,11 +401,14 @@ struct LaunchImage {
     q.buckets = buckets.empty() ? nullptr
                                 : reinterpret_cast<const VolBucket*>(d_block + sizeof(DevParams) +
                                                                      cams.size() * sizeof(CamDev));
+    const size_t pin_off = sizeof(DevParams) + cams.size() * sizeof(CamDev) + buckets.size() * sizeof(VolBucket);
+    q.pin = pin.empty() ? nullptr : reinterpret_cast<const PinholeDev*>(d_block + pin_off);
     std::memcpy(dst, &q, sizeof(q));
     if (!cams.empty()) std::memcpy(dst + sizeof(DevParams), cams.data(), cams.size() * sizeof(CamDev));
     if (!buckets.empty())
       std::memcpy(dst + sizeof(DevParams) + cams.size() * sizeof(CamDev), buckets.data(),
                   buckets.size() * sizeof(VolBucket));
+    if (!pin.empty()) std::memcpy(dst + pin_off, pin.data(), sizeof(PinholeDev));
   }
   // Grid of the persistent kernel.
   int grid() const {
@@ -582,14 +588,8 @@ struct StreamCtx {
 std::mutex g_mu;
 std::map<std::pair<int, void*>, StreamCtx> g_ctx;
 
-int trace_launch(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts, const double* disk_luts,
-                 int disk_lut_n, const BH_Volume* vol, const BH_VolumeSet* vs, const BH_RaySpec* rays,
-                 long long n_rays, const int64_t* ids, const uint64_t* n_ids_dev, long long max_ids,
-                 BH_PixelRecord* out, void* stream, bool compact, const char* who) {
-  LaunchImage im;
-  if (build_launch(cams, n_cams, opts, disk_luts, disk_lut_n, vol, vs, rays, n_rays, ids, n_ids_dev, max_ids, out,
-                   compact, who, &im) != 0)
-    return -1;
+// Uploads a launch image into the stream's context block and launches it.
+int launch_on_stream(const LaunchImage& im, void* stream, const char* who) {
   hipStream_t s = static_cast<hipStream_t>(stream);
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return fail(std::string(who) + ": no HIP device");
@@ -624,6 +624,29 @@ int trace_launch(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
   if (launch_image(im, reinterpret_cast<const DevParams*>(ctx->d_block), ctx->d_head, s, who) != 0) return -1;
   (void)hipEventRecord(g_ev1, s);
   return 0;
+}
+
+int trace_launch(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts, const double* disk_luts,
+                 int disk_lut_n, const BH_Volume* vol, const BH_VolumeSet* vs, const BH_RaySpec* rays,
+                 long long n_rays, const int64_t* ids, const uint64_t* n_ids_dev, long long max_ids,
+                 BH_PixelRecord* out, void* stream, bool compact, const char* who) {
+  LaunchImage im;
+  if (build_launch(cams, n_cams, opts, disk_luts, disk_lut_n, vol, vs, rays, n_rays, ids, n_ids_dev, max_ids, out,
+                   compact, who, &im) != 0)
+    return -1;
+  return launch_on_stream(im, stream, who);
+}
+
+int trace_pinhole(const BH_Camera& cam, const BH_TraceOptions& opts, const double* disk_lut, int lut_n,
+                  const PinholeDev& pin, double r_isco, BH_PixelRecord* out, void* stream, const char* who) {
+  LaunchImage im;
+  if (build_launch(&cam, 1, &opts, disk_lut, lut_n, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0, out, false,
+                   who, &im) != 0)
+    return -1;
+  if (opts.volume_enabled) return fail(std::string(who) + ": the pinhole camera takes no volume");
+  if (r_isco > 0.0) im.cams[0].r_isco = r_isco;
+  im.pin.push_back(pin);
+  return launch_on_stream(im, stream, who);
 }
 
 }  // namespace bhk

@@ -36,6 +36,7 @@ struct DevParams {
   int n_cams, lut_n, batch, max_steps, max_attempts, max_crossings, disk_enabled;
   int compact_ids;  // id list: record of entry q -> out[q] instead of out[id]
   const VolBucket* buckets;  // volume set (camera k samples buckets[k.vol_index]); NULL: P.vol
+  const PinholeDev* pin;     // pinhole camera (render launch): camera 0's pixels start from it
   // Ray sources other than every pixel of the cameras: an explicit ray list
   // (rays, n_total entries, record i -> out[i]) or a list of pixel ids
   // (ids, count read from count_dev, record -> out[id]).
@@ -140,6 +141,12 @@ __device__ __forceinline__ long long refill_id(bool need, unsigned long long* he
 // -1 with the error set, prefixed by who, on an invalid camera.
 int make_cam_devs(const BH_Camera* cams, int n_cams, double horizon_factor, const char* who,
                   std::vector<CamDev>* out, long long* total);
+
+// Host: trace the pixels of one pinhole camera (cam supplies the spin, the
+// horizon stop and the pixel grid; r_isco > 0 overrides the disk's inner
+// edge) on `stream` (trace.hip).
+int trace_pinhole(const BH_Camera& cam, const BH_TraceOptions& opts, const double* disk_lut, int lut_n,
+                  const PinholeDev& pin, double r_isco, BH_PixelRecord* out, void* stream, const char* who);
 
 // FP32 precision form (trace_fp32.hip): launch on `grid` workgroups, and
 // the workgroups per CU its occupancy allows.

@@ -52,7 +52,9 @@ __device__ __forceinline__ bool lane_setup(LaneF& L, const DevParams& P, long lo
   double alpha, beta;
   if (!resolve_ray(P, q, &L.gid, &L.cam, &k, &alpha, &beta)) return false;
   double yd[kN], fd[kN];
-  L.cd = setup_ray_ab(k, alpha, beta, yd);  // FP64 island: constants of motion
+  // FP64 island: constants of motion (a pinhole camera's table entry maps
+  // pixels to their centres)
+  L.cd = P.pin ? setup_pinhole(*P.pin, k.a, alpha, beta, yd) : setup_ray_ab(k, alpha, beta, yd);
   L.cf = convert_const<float>(L.cd);
   rhs(L.cd, yd, fd);
 #pragma unroll

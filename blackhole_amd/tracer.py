@@ -261,3 +261,51 @@ class Frame:
             self.close()
         except Exception:
             pass
+
+
+def launch_params(rs: float = 2.0, spin: float = 0.9, cam_pos=(0.0, -20.0, 12.0), look_at=(0.0, 0.0, 0.0),
+                  fov_scale: float = 0.35, width: int = 64, height: int = 48, max_steps: int = 20000,
+                  max_dist: float = 400.0, isco: float = 0.0, **kw) -> abi.BH_LaunchParams:
+    """A BH_LaunchParams for a pinhole camera at cam_pos looking at look_at
+    (z = spin axis); basis columns [right, up, forward] with up toward +z."""
+    p = abi.BH_LaunchParams()
+    p.rs, p.spin, p.isco, p.step_size, p.fov_scale, p.max_dist = rs, spin, isco, 0.1, fov_scale, max_dist
+    pos = np.array(cam_pos, dtype=np.float64)
+    fwd = np.array(look_at, dtype=np.float64) - pos
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, np.array([0.0, 0.0, 1.0]))
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    basis = np.concatenate([right, up, fwd])  # column-major: columns right, up, forward
+    for i in range(3):
+        p.cam_pos[i] = pos[i]
+    for i in range(9):
+        p.cam_basis[i] = basis[i]
+    p.max_steps, p.width, p.height = max_steps, width, height
+    p.adisk_enabled, p.redshift_enabled, p.kerr_enabled, p.adisk_lit = 1, 1, 1, 1.0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def trace_launch_params(params: abi.BH_LaunchParams, variant: int = 2, out=None, stream=None):
+    """Records (uint8 [w*h, 128] cuda) of a render launch (bh_trace_launch_params)."""
+    torch = _torch()
+    n = params.width * params.height
+    if out is None:
+        out = alloc_records(n)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib().bh_trace_launch_params(C.byref(params), variant, C.c_void_p(out.data_ptr()),
+                                       C.c_void_p(s.cuda_stream)), "bh_trace_launch_params")
+    return out
+
+
+def render_launch(params: abi.BH_LaunchParams, variant: int = -1, fb=None, stream=None):
+    """Shaded float4 framebuffer (float32 [h, w, 4] cuda) of bh_launch_geodesic_kernel."""
+    torch = _torch()
+    if fb is None:
+        fb = torch.empty((params.height, params.width, 4), dtype=torch.float32, device="cuda")
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib().bh_launch_geodesic_kernel(C.c_void_p(fb.data_ptr()), C.byref(params), variant,
+                                          C.c_void_p(s.cuda_stream)), "bh_launch_geodesic_kernel")
+    return fb

@@ -194,10 +194,21 @@ inline void project_radial(const RayConst<T>& c, T* y, T* f) {
 // sin/cos of the inclination come from the host (one libm call per camera)
 // so CPU and GPU start from identical bits.
 // ---------------------------------------------------------------------------
+// Pinhole camera at a finite position (the render-launch camera): the
+// camera's Boyer-Lindquist position, the flat spherical basis there, the
+// column-major camera basis and the pixel mapping (host values).
+struct PinholeConst {
+  double r, cos_t, sin_t, phi;
+  double e_r[3], e_t[3], e_p[3];
+  double basis[9];
+  double fov, aspect, shift_x, shift_y, inv_w, inv_h;
+};
+
 struct CamConst {
   double a, sin_o, cos_o, r_obs, r_stop, r_isco;
   double alpha_min, dalpha, beta_max, dbeta;
   int64_t w, h, offset;
+  const PinholeConst* pin = nullptr;  // pinhole camera: pixels start from it
 };
 
 inline CamConst make_cam_const(const BH_Camera& cam, const BH_TraceOptions& o) {
@@ -218,8 +229,63 @@ inline CamConst make_cam_const(const BH_Camera& cam, const BH_TraceOptions& o) {
   return k;
 }
 
+// Ray through pixel centre (xc, yc) of a pinhole camera: the direction as
+// d_ray_dir maps pixels (src/cuda/device_physics.cuh:1953-1967), its
+// projections on the flat spherical basis taken as k^r, k^theta, k^phi, k^t
+// from the null condition of the Kerr metric, E, L and p_theta as in
+// d_kerr_init_consts (device_physics.cuh:316-392), normalised to E = 1.
+// Magnitudes of the initial Mino velocities from the potentials, signs from
+// k^r and -k^theta.  Requires r > 2 (k^t = (hb + sqrt(disc)) / -g_tt).
+inline RayConst<double> setup_pinhole(const PinholeConst& p, double a, double xc, double yc, double* y) {
+  double u = (2.0 * xc * p.inv_w - 1.0) * p.fov;
+  u = u * p.aspect + p.shift_x;
+  const double v = (2.0 * yc * p.inv_h - 1.0) * p.fov + p.shift_y;
+  const double lx = -u, ly = -v, lz = 1.0;
+  const double inv_n = 1.0 / std::sqrt(lx * lx + ly * ly + lz * lz);
+  double d[3];
+  for (int i = 0; i < 3; ++i) d[i] = (p.basis[i] * lx + p.basis[3 + i] * ly + p.basis[6 + i] * lz) * inv_n;
+  const double r = p.r;
+  const double kr = d[0] * p.e_r[0] + d[1] * p.e_r[1] + d[2] * p.e_r[2];
+  const double kth = (d[0] * p.e_t[0] + d[1] * p.e_t[1] + d[2] * p.e_t[2]) / r;
+  const double kph = (d[0] * p.e_p[0] + d[1] * p.e_p[1] + d[2] * p.e_p[2]) / (r * p.sin_t);
+  const double cos2 = p.cos_t * p.cos_t, sin2 = p.sin_t * p.sin_t, a2 = a * a;
+  const double sigma = r * r + a2 * cos2;
+  const double delta = r * r - 2.0 * r + a2;
+  const double f = 2.0 * r / sigma;
+  const double g_tt = -(1.0 - f);
+  const double g_tp = -f * a * sin2;
+  const double g_rr = sigma / delta;
+  const double g_pp = (r * r + a2 + f * a2 * sin2) * sin2;
+  const double spatial = g_rr * kr * kr + sigma * kth * kth + g_pp * kph * kph;
+  const double hb = g_tp * kph;
+  const double disc = hb * hb - g_tt * spatial;
+  const double kt = (hb + std::sqrt(disc)) / (-g_tt);
+  const double E = -(g_tt * kt + g_tp * kph);
+  const double L = g_tp * kt + g_pp * kph;
+  const double xi = L / E;
+  const double pth = sigma * kth / E;
+  const double eta = pth * pth - a2 * cos2 + xi * xi * cos2 / sin2;
+  const RayConst<double> c = make_const(a, xi, eta);
+  y[0] = r;
+  y[1] = p.cos_t;
+  y[2] = p.phi;
+  double R = radial_potential(c, r);
+  double U = polar_potential(c, p.cos_t);
+  if (R < 0.0) R = 0.0;
+  if (U < 0.0) U = 0.0;
+  y[3] = kr >= 0.0 ? std::sqrt(R) : -std::sqrt(R);
+  y[4] = kth >= 0.0 ? -std::sqrt(U) : std::sqrt(U);
+  return c;
+}
+
 template <class T>
-inline RayConst<T> setup_ray(const CamConst& k, int64_t px, int64_t py, T* y) {
+inline RayConst<T> setup_ray(const CamConst& k, int64_t px, int64_t py, T* y);
+
+template <>
+inline RayConst<double> setup_ray<double>(const CamConst& k, int64_t px, int64_t py, double* y);
+
+template <class T>
+inline RayConst<T> setup_ray_image_plane(const CamConst& k, int64_t px, int64_t py, T* y) {
   const T alpha = T(k.alpha_min) + (T(static_cast<double>(px)) + T(0.5)) * T(k.dalpha);
   const T beta = T(k.beta_max) - (T(static_cast<double>(py)) + T(0.5)) * T(k.dbeta);
   const T a = T(k.a);
@@ -234,6 +300,17 @@ inline RayConst<T> setup_ray(const CamConst& k, int64_t px, int64_t py, T* y) {
   y[3] = -sqrt_(R);
   y[4] = T(k.sin_o) * beta;
   return c;
+}
+
+template <class T>
+inline RayConst<T> setup_ray(const CamConst& k, int64_t px, int64_t py, T* y) {
+  return setup_ray_image_plane<T>(k, px, py, y);
+}
+
+template <>
+inline RayConst<double> setup_ray<double>(const CamConst& k, int64_t px, int64_t py, double* y) {
+  if (k.pin) return setup_pinhole(*k.pin, k.a, static_cast<double>(px) + 0.5, static_cast<double>(py) + 0.5, y);
+  return setup_ray_image_plane<double>(k, px, py, y);
 }
 
 // ---------------------------------------------------------------------------

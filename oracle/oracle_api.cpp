@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "kerr_oracle.hpp"
+#include "../include/bh_kernel_launch.h"
 
 using namespace orc;
 
@@ -250,6 +251,145 @@ int orc_trace(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
   for (int t = 1; t < nthreads; ++t) th.emplace_back(worker);
   worker();
   for (auto& t : th) t.join();
+  return 0;
+}
+
+// Trace every pixel of a render launch (struct BH_LaunchParams) in FP64 as
+// the render ABI maps it: lengths in M = rs/2, the pinhole camera at cam_pos
+// (off the spin axis, r > 2), the disk from isco (or the prograde ISCO when
+// isco <= 0) to 30 rs, escape beyond max_dist, DP45 rtol 1e-8 / atol 1e-10,
+// h_init 0.01, h_max 0.1, max_steps accepted steps and 4 max_steps + 100
+// attempts, stop at 1.01 r_+.  out: width x height records, row-major.
+int orc_disk_lut(double a, double r_in, double r_out, int n, double* out);
+
+int orc_trace_launch_params(const BH_LaunchParams* lp, BH_PixelRecord* out, int nthreads) {
+  if (!lp || !out || lp->width <= 0 || lp->height <= 0 || !(lp->rs > 0.0f)) return -1;
+  const BH_LaunchParams& p = *lp;
+  const double M = 0.5 * static_cast<double>(p.rs);
+  const double x = static_cast<double>(p.cam_pos[0]) / M;
+  const double y = static_cast<double>(p.cam_pos[1]) / M;
+  const double z = static_cast<double>(p.cam_pos[2]) / M;
+  const double r = std::sqrt(x * x + y * y + z * z);
+  const double rho = std::sqrt(x * x + y * y);
+  if (!(rho > 0.0) || !(r > 2.0)) return -1;
+  PinholeConst pin;
+  pin.r = r;
+  pin.cos_t = z / r;
+  pin.sin_t = rho / r;
+  pin.phi = std::atan2(y, x);
+  const double cp = x / rho, sp = y / rho;
+  pin.e_r[0] = x / r;
+  pin.e_r[1] = y / r;
+  pin.e_r[2] = z / r;
+  pin.e_t[0] = pin.cos_t * cp;
+  pin.e_t[1] = pin.cos_t * sp;
+  pin.e_t[2] = -pin.sin_t;
+  pin.e_p[0] = -sp;
+  pin.e_p[1] = cp;
+  pin.e_p[2] = 0.0;
+  for (int i = 0; i < 9; ++i) pin.basis[i] = static_cast<double>(p.cam_basis[i]);
+  pin.fov = static_cast<double>(p.fov_scale);
+  pin.aspect = static_cast<double>(p.width) / static_cast<double>(p.height);
+  pin.shift_x = static_cast<double>(p.frame_shift_x);
+  pin.shift_y = static_cast<double>(p.frame_shift_y);
+  pin.inv_w = 1.0 / static_cast<double>(p.width);
+  pin.inv_h = 1.0 / static_cast<double>(p.height);
+  BH_TraceOptions o;
+  std::memset(&o, 0, sizeof(o));
+  o.integrator = BH_INTEGRATOR_DP45;
+  o.precision = BH_PRECISION_FP64;
+  o.max_steps = p.max_steps;
+  o.max_attempts = 4 * p.max_steps + 100;
+  o.disk_enabled = p.adisk_enabled ? 1 : 0;
+  o.max_crossings = 3;
+  o.h_init = 0.01;
+  o.h_max = 0.1;
+  o.h_vol = 0.01;
+  o.rtol = 1e-8;
+  o.atol = 1e-10;
+  o.horizon_factor = 1.01;
+  o.r_escape = static_cast<double>(p.max_dist) / M;
+  o.disk_r_out = 30.0 * static_cast<double>(p.rs) / M;
+  o.nu_obs = 230e9;
+  o.disk_T_max = 1e7;
+  const double spin = static_cast<double>(p.spin);
+  CamConst k;
+  k.a = spin;
+  k.sin_o = pin.sin_t;
+  k.cos_o = pin.cos_t;
+  k.r_obs = r;
+  k.r_stop = o.horizon_factor * outer_horizon(spin);
+  k.r_isco = p.isco > 0.0f ? static_cast<double>(p.isco) / M : isco_prograde(spin);
+  k.alpha_min = 0.0;
+  k.dalpha = 1.0;
+  k.beta_max = 0.0;
+  k.dbeta = -1.0;
+  k.w = p.width;
+  k.h = p.height;
+  k.offset = 0;
+  k.pin = &pin;
+  std::vector<double> lut(256);
+  if (orc_disk_lut(spin, k.r_isco, o.disk_r_out, 256, lut.data()) != 0) return -1;
+  Scene S;
+  S.o = &o;
+  S.luts = lut.data();
+  S.lut_n = 256;
+  S.vol_on = false;
+  const int64_t n = int64_t(p.width) * p.height;
+  if (nthreads <= 0) nthreads = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+  std::atomic<int64_t> next{0};
+  auto worker = [&]() {
+    for (;;) {
+      const int64_t i0 = next.fetch_add(64);
+      if (i0 >= n) break;
+      const int64_t i1 = std::min<int64_t>(n, i0 + 64);
+      for (int64_t i = i0; i < i1; ++i) trace_dp45(S, k, i % k.w, i / k.w, &out[i]);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nthreads; ++t) th.emplace_back(worker);
+  worker();
+  for (auto& t : th) t.join();
+  return 0;
+}
+
+// Setup of one pinhole pixel of a render launch: y[5] and {xi, eta}.
+int orc_pinhole_setup(const BH_LaunchParams* lp, int64_t px, int64_t py, double* y, double* xi_eta) {
+  if (!lp) return -1;
+  const BH_LaunchParams& p = *lp;
+  const double M = 0.5 * static_cast<double>(p.rs);
+  const double x = static_cast<double>(p.cam_pos[0]) / M;
+  const double yy = static_cast<double>(p.cam_pos[1]) / M;
+  const double z = static_cast<double>(p.cam_pos[2]) / M;
+  const double r = std::sqrt(x * x + yy * yy + z * z);
+  const double rho = std::sqrt(x * x + yy * yy);
+  if (!(rho > 0.0) || !(r > 2.0)) return -1;
+  PinholeConst pin;
+  pin.r = r;
+  pin.cos_t = z / r;
+  pin.sin_t = rho / r;
+  pin.phi = std::atan2(yy, x);
+  const double cp = x / rho, sp = yy / rho;
+  pin.e_r[0] = x / r;
+  pin.e_r[1] = yy / r;
+  pin.e_r[2] = z / r;
+  pin.e_t[0] = pin.cos_t * cp;
+  pin.e_t[1] = pin.cos_t * sp;
+  pin.e_t[2] = -pin.sin_t;
+  pin.e_p[0] = -sp;
+  pin.e_p[1] = cp;
+  pin.e_p[2] = 0.0;
+  for (int i = 0; i < 9; ++i) pin.basis[i] = static_cast<double>(p.cam_basis[i]);
+  pin.fov = static_cast<double>(p.fov_scale);
+  pin.aspect = static_cast<double>(p.width) / static_cast<double>(p.height);
+  pin.shift_x = static_cast<double>(p.frame_shift_x);
+  pin.shift_y = static_cast<double>(p.frame_shift_y);
+  pin.inv_w = 1.0 / static_cast<double>(p.width);
+  pin.inv_h = 1.0 / static_cast<double>(p.height);
+  const RayConst<double> c =
+      setup_pinhole(pin, static_cast<double>(p.spin), static_cast<double>(px) + 0.5, static_cast<double>(py) + 0.5, y);
+  xi_eta[0] = c.xi;
+  xi_eta[1] = c.eta;
   return 0;
 }
 

@@ -78,6 +78,10 @@ def lib():
         L.orc_bessel_k.restype = _D
         L.orc_k2_table.argtypes = [C.c_int, _D, _D, _PD]
         L.orc_rte_step.argtypes = [_PD, _D, _D, _D]
+        L.orc_trace_launch_params.argtypes = [C.POINTER(abi.BH_LaunchParams), C.c_void_p, C.c_int]
+        L.orc_trace_launch_params.restype = C.c_int
+        L.orc_pinhole_setup.argtypes = [C.POINTER(abi.BH_LaunchParams), C.c_int64, C.c_int64, _PD, _PD]
+        L.orc_pinhole_setup.restype = C.c_int
         _lib = L
     return _lib
 
@@ -120,3 +124,17 @@ def trace(cams, opts, luts=None, lut_n=0, vol=None, cells=None, ids=None,
     )
     assert rc == 0
     return out
+
+
+def trace_launch_params(params, nthreads=0) -> np.ndarray:
+    """Oracle records of a render launch (width x height, row-major)."""
+    out = np.zeros(params.width * params.height, dtype=abi.RECORD_DTYPE)
+    assert lib().orc_trace_launch_params(C.byref(params), out.ctypes.data_as(C.c_void_p), nthreads) == 0
+    return out
+
+
+def pinhole_setup(params, px, py):
+    y = np.zeros(5)
+    xe = np.zeros(2)
+    assert lib().orc_pinhole_setup(C.byref(params), px, py, dptr(y), dptr(xe)) == 0
+    return y, xe[0], xe[1]
