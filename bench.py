@@ -21,10 +21,14 @@ intensity error over pixels with I_ref > 1e-6 max(I_ref), photon-ring
 the fastest method that meets the gate (mean relative intensity error <=
 1e-3 and ring-diameter error <= 0.1 %); both methods' rates are reported.
 
-``--workload c2`` times the volumetric FP64 DP45 tracer on the c2 frame
-(1024^2, a = 0.94, 17 deg, torus 256 x 128 x 64, 230 GHz) instead.
+Other workloads (one JSON line each, same contract):
+  --workload c4  the batched form: 256 cameras, spin-bucket tori, one
+                 persistent launch replayed from a hipGraph, autotuned shape;
+  --workload c3  the 4096^2 precision form (FP32 + FP64 islands) against FP64;
+  --workload c2  the volumetric FP64 DP45 tracer on the c2 frame (1024^2,
+                 a = 0.94, 17 deg, torus 256 x 128 x 64, 230 GHz).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c4|c3|c2]
 """
 from __future__ import annotations
 
@@ -126,9 +130,9 @@ def frame_accuracy(sur_rec, plain_rec, cams):
 
 
 def device_frame_flops(rec):
-    """flops.frame_flops of a DP45 thin-disk frame from the device records
-    (uint8 [n, 128]: steps and attempts are int32 words 26, 27; hit class
-    and crossing count bytes 112, 113)."""
+    """flops.frame_flops of a DP45 frame from the device records (uint8
+    [n, 128]: steps, attempts and volume samples are int32 words 26, 27, 31;
+    hit class and crossing count bytes 112, 113)."""
     import torch
     from blackhole_amd import abi, flops
     w = rec.view(torch.int32)
@@ -138,8 +142,11 @@ def device_frame_flops(rec):
     n_valid = float((cls != 0).sum())
     ncross = float(rec[:, 113].double().sum())
     disk = float((cls == abi.HIT_DISK).sum())
+    samples = float(w[:, 31].double().sum())
     f = (attempts * flops.DP45_ATTEMPT + steps * flops.PROJECTION + n_valid * flops.SETUP
          + ncross * flops.CROSSING + disk * flops.DISK_HIT)
+    if samples > 0:
+        f += samples * flops.VOLUME_SAMPLE + (steps - samples) * flops.VOLUME_STEP_NOSAMPLE
     return f, attempts
 
 
@@ -256,6 +263,130 @@ def bench_c5(args):
     return res
 
 
+def bench_c4(args):
+    """Batched form: 256 cameras (48.7 M rays) of mixed resolution, spin,
+    inclination and distance, thin disk + one torus per spin bucket, in one
+    persistent launch; the launch shape autotuned beforehand (excluded), the
+    frame replayed from its hipGraph in the timed region."""
+    import numpy as np
+    import torch
+    from blackhole_amd import tracer, workloads
+
+    w = workloads.c4()
+    sc = tracer.upload_scene(w["cams"], w["opts"], luts=w["luts"])
+    vset = tracer.VolumeSetDevice(w["cells_list"], w["vols"], w["cam_vol"])
+    n = sc.n_records
+    out = tracer.alloc_records(n)
+    fr = tracer.Frame(sc, out, vset)
+    t0 = time.perf_counter()
+    tune = fr.autotune(sample_stride=16)
+    tune_s = time.perf_counter() - t0
+    stream = torch.cuda.current_stream()
+    graph_ms, graph_ev = time_steps(lambda: fr.graph_launch(stream), args.steps, args.warmup)
+    eager_ms, eager_ev = time_steps(lambda: fr.launch(stream), args.steps, 0)
+    f, attempts = device_frame_flops(out)
+    achieved = f / (graph_ev / 1000.0) / 1e12
+    samples = float(out.view(torch.int32)[:, 31].double().sum())
+    res = {
+        "metric": METRIC, "value": n / (graph_ms / 1000.0), "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": graph_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded cameras and spin-bucket tori)",
+        "config": {"workload": "c4: 256 cameras (side logU 64-1024, a U[0,0.998], incl U[5,85] deg, r_obs "
+                               "U[50,1000]), thin disk + 8 spin-bucket tori 128x64x32, DP45 FP64, 230 GHz, "
+                               "one persistent launch, hipGraph replay", "rays": n},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_trace_fp64<DP45, volume, volume set>", "kernel_ms": graph_ev,
+                     "flop_per_launch": f},
+        "autotune": {"blocks_per_cu": tune.blocks_per_cu, "refill_batch": tune.refill_batch,
+                     "occupancy_limit": tune.occupancy_limit, "sample_rays": tune.sample_rays,
+                     "best_sample_ms": tune.sample_ms, "tuning_s": tune_s},
+        "graph_vs_eager": {"graph_ms": graph_ms, "eager_ms": eager_ms, "graph_event_ms": graph_ev,
+                           "eager_event_ms": eager_ev},
+        "frame_stats": {"attempts_mean": attempts / n, "samples_mean": samples / n,
+                        "volume_GBps_algorithmic": samples * 128 / (graph_ev / 1000.0) / 1e9},
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        from parity_util import compare
+        rng = np.random.default_rng(20261204)
+        m = min(args.cpu_sample // 8, n)
+        ids = np.sort(rng.choice(n, size=m, replace=False)).astype(np.int64)
+        offs = np.array([c.record_offset for c in w["cams"]])
+        cam_of = np.searchsorted(offs, ids, side="right") - 1
+        gpu = tracer.records_to_numpy(out[torch.from_numpy(ids).cuda()])
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        pars = []
+        for b in range(len(w["vols"])):
+            sel = np.nonzero(np.isin(cam_of, [i for i, v in enumerate(w["cam_vol"]) if v == b]))[0]
+            if sel.size == 0:
+                continue
+            ref = oracle_lib.trace(w["cams"], w["opts"], luts=w["luts"], lut_n=w["luts"].shape[1], vol=w["vols"][b],
+                                   cells=w["cells_list"][b].reshape(-1), ids=ids[sel], nthreads=threads)
+            pars.append(compare(gpu[sel], ref))
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": m / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+                               "sample": f"{m} seeded random pixels of the 256 c4 cameras ({dt:.1f} s wall on "
+                                         f"{threads} threads, FP64 oracle per spin bucket)"}
+        res["plain_gpu_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+        res["parity_vs_oracle_on_sample"] = {
+            "n": sum(p["n"] for p in pars), "class_mismatch": sum(p["class_mismatch"] for p in pars),
+            "bitwise_exit": min(p["bitwise_exit"] for p in pars),
+            "max_rel_intensity": max(p["max_rel_intensity"] for p in pars)}
+    fr.close()
+    return res
+
+
+def bench_c3(args):
+    """Precision form: the 4096^2 a = 0.998, 85 deg frame (thin disk + torus,
+    DP45 rtol 1e-6) in FP32 with FP64 islands, against the same frame in
+    FP64: rays/s of both and the FP32 image's errors."""
+    import math
+    import torch
+    from blackhole_amd import abi, analysis, tracer, workloads
+
+    size = args.size if args.size != 2048 else 4096
+    w = workloads.c3(size, size)
+    res_legs = {}
+    recs = {}
+    for prec in ("fp32", "fp64"):
+        w["opts"].precision = abi.PRECISION_FP32 if prec == "fp32" else abi.PRECISION_FP64
+        sc = tracer.upload_scene(w["cams"], w["opts"], luts=w["luts"], cells=w["cells"], vol=w["vol"])
+        out = tracer.alloc_records(sc.n_records)
+        ms, ev = time_steps(lambda: tracer.launch(sc, out), args.steps, args.warmup)
+        tracer.launch(sc, out)
+        torch.cuda.synchronize()
+        k_ms = tracer.last_kernel_ms()
+        recs[prec] = out
+        res_legs[prec] = {"rays_per_s": sc.n_records / (ms / 1000.0), "ms_per_step": ms, "kernel_ms": k_ms}
+    n = recs["fp64"].shape[0]
+    acc = frame_accuracy(recs["fp32"], recs["fp64"], w["cams"])
+    islands = float((recs["fp32"][:, 114] & abi.FLAG_FP64_ISLAND != 0).double().mean())
+    w32 = recs["fp32"].view(torch.int32)
+    fp64_frac = float(w32[:, 30].double().sum() / w32[:, 26].double().sum())
+    f, _ = device_frame_flops(recs["fp32"])
+    achieved = f / (res_legs["fp32"]["kernel_ms"] / 1000.0) / 1e12
+    return {
+        "metric": METRIC, "value": res_legs["fp32"]["rays_per_s"], "unit": "rays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": res_legs["fp32"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded torus volume, analytic camera)",
+        "config": {"workload": f"c3: a=0.998, 85 deg, {size}x{size}, thin disk + torus 256x128x64, DP45 rtol 1e-6 "
+                               f"FP32 with FP64 islands (r < 1.2 r_+, turning points within 0.05 M)", "rays": n},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": achieved / 157.3, "traffic": None, "kernel": "k_trace_fp32<DP45, volume>",
+                     "kernel_ms": res_legs["fp32"]["kernel_ms"], "flop_per_launch": f,
+                     "note": "FP32 peak assumes packed FP32, which this build disables; scalar FP32 issues at "
+                             "the 78.6 TF FP64 rate"},
+        "fp32": res_legs["fp32"], "fp64": res_legs["fp64"],
+        "fp32_vs_fp64": {k: v for k, v in acc.items() if k != "frames"},
+        "rays_with_fp64_islands": islands, "fp64_step_fraction": fp64_frac,
+        "critical_curve_band_M": 0.1,
+    }
+
+
 def bench_c2(args):
     import numpy as np
     import torch
@@ -328,7 +459,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c5", "c2"], default="c5")
+    ap.add_argument("--workload", choices=["c5", "c4", "c3", "c2"], default="c5")
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=8_000_000, help="c5: oracle pixels for the CPU baseline")
@@ -345,7 +476,7 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
-    res = bench_c5(args) if args.workload == "c5" else bench_c2(args)
+    res = {"c5": bench_c5, "c4": bench_c4, "c3": bench_c3, "c2": bench_c2}[args.workload](args)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -39,7 +39,6 @@ struct Lane {
   double y[kN], k1[kN];
   double h;
   RtAcc rt;
-  double cr0, cr1, cr2, cg0, cg1, cg2;
   double r_stop, r_isco;
   long long gid;
   int cam;
@@ -65,8 +64,7 @@ __device__ __forceinline__ bool lane_setup(Lane& L, const DevParams& P, long lon
   L.rt.a0 = 0.0;
   L.rt.e0 = 0.0;
   L.rt.s0 = VOL ? affine_rate(L.c, L.y[0], L.y[1]) : 0.0;
-  L.cr0 = L.cr1 = L.cr2 = 0.0;
-  L.cg0 = L.cg1 = L.cg2 = 0.0;
+  clear_crossing_slots(P.out + L.gid);
   L.r_stop = k.r_stop;
   L.r_isco = k.r_isco;
   L.vb = k.vol_index;
@@ -98,16 +96,7 @@ __device__ __forceinline__ bool crossing(Lane& L, const DevParams& P, const doub
   if ((y0[1] < 0.0) == (y1[1] < 0.0)) return false;
   const double rc = crossing_radius(y0, f0, y1, f1, h);
   const double g = (rc >= L.r_isco) ? disk_g(rc, L.c.a, L.c.xi) : 0.0;
-  // Branch-free slot update keeps the six slots in registers (an if/else
-  // chain on ncross is lowered to a dynamically indexed scratch array).
-  const bool rec_ok = L.ncross < P.max_crossings;
-  const bool s0 = rec_ok && L.ncross == 0, s1 = rec_ok && L.ncross == 1, s2 = rec_ok && L.ncross == 2;
-  L.cr0 = s0 ? rc : L.cr0;
-  L.cg0 = s0 ? g : L.cg0;
-  L.cr1 = s1 ? rc : L.cr1;
-  L.cg1 = s1 ? g : L.cg1;
-  L.cr2 = s2 ? rc : L.cr2;
-  L.cg2 = s2 ? g : L.cg2;
+  if (L.ncross < P.max_crossings) store_crossing(P.out + L.gid, L.ncross, rc, g);
   if (L.ncross < 255) L.ncross++;
   if (P.disk_enabled && rc >= L.r_isco && rc <= P.disk_r_out) {
     const double* lut = P.luts + static_cast<long long>(P.cams[L.cam].lut_index) * P.lut_n;
@@ -135,9 +124,6 @@ __device__ __forceinline__ void write_record(const Lane& L, const DevParams& P, 
   double2* d2 = reinterpret_cast<double2*>(rec);
   d2[0] = make_double2(L.rt.I, L.y[0]);
   d2[1] = make_double2(L.y[1], L.y[2]);
-  d2[2] = make_double2(L.cr0, L.cr1);
-  d2[3] = make_double2(L.cr2, L.cg0);
-  d2[4] = make_double2(L.cg1, L.cg2);
   d2[5] = make_double2(L.rt.tau, __builtin_fabs(__builtin_fma(L.y[3], L.y[3], -R)) / (rs * rs));
   const double dc = __builtin_fabs(__builtin_fma(L.y[4], L.y[4], -U)) /
                     (__builtin_fabs(L.c.eta) + L.c.a2 + L.c.xi * L.c.xi);
@@ -204,8 +190,11 @@ __device__ __forceinline__ int step_dp45(Lane& L, const DevParams& P) {
   return 0;
 }
 
+// Two workgroups per CU: the volume variants would otherwise take a few
+// registers past the 256 that two waves per SIMD allow (the handful of
+// spilled values cost less than the lost latency hiding).
 template <int INTEG, bool VOL, bool MV>
-__global__ void __launch_bounds__(kBlock) k_trace_fp64(const DevParams* __restrict__ Pp) {
+__global__ void __launch_bounds__(kBlock, 2) k_trace_fp64(const DevParams* __restrict__ Pp) {
   const DevParams& P = *Pp;
   Lane L;
   L.gid = -1;

@@ -104,6 +104,22 @@ __device__ __forceinline__ bool resolve_ray(const DevParams& P, long long q, lon
   return true;
 }
 
+// The crossing slots live in the ray's output record, not in registers (six
+// doubles per lane would cost the volume kernels their second wave per SIMD):
+// zeroed when the lane takes the ray, each recorded crossing stored as it
+// happens (at most max_crossings <= 3 stores per ray).
+__device__ __forceinline__ void clear_crossing_slots(BH_PixelRecord* rec) {
+  double2* d2 = reinterpret_cast<double2*>(rec);
+  d2[2] = make_double2(0.0, 0.0);
+  d2[3] = make_double2(0.0, 0.0);
+  d2[4] = make_double2(0.0, 0.0);
+}
+
+__device__ __forceinline__ void store_crossing(BH_PixelRecord* rec, int slot, double rc, double g) {
+  rec->cross_r[slot] = rc;
+  rec->cross_g[slot] = g;
+}
+
 // Number of queue entries (an id-list count lives in device memory so that
 // a producer kernel can append ids without a host round trip).
 __device__ __forceinline__ long long queue_length(const DevParams& P) {

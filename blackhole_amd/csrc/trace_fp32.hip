@@ -27,7 +27,6 @@ struct LaneF {
   float y[kN], cy[kN], k1[kN];
   float h;
   float I, Ic, Tr, tau, tauc, a0, e0, s0;
-  double cr0, cr1, cr2, cg0, cg1, cg2;
   float r_stop, r_isco, island_r;
   long long gid;
   int cam, steps, attempts, ncross, nsamp, fp64_steps;
@@ -69,8 +68,7 @@ __device__ __forceinline__ bool lane_setup(LaneF& L, const DevParams& P, long lo
   L.tau = L.tauc = 0.0f;
   L.a0 = L.e0 = 0.0f;
   L.s0 = VOL ? affine_rate_f(L.cf, L.y[0], L.y[1]) : 0.0f;
-  L.cr0 = L.cr1 = L.cr2 = 0.0;
-  L.cg0 = L.cg1 = L.cg2 = 0.0;
+  clear_crossing_slots(P.out + L.gid);
   L.r_stop = static_cast<float>(k.r_stop);
   L.r_isco = static_cast<float>(k.r_isco);
   L.island_r = static_cast<float>(P.island_r_factor * k.r_plus);
@@ -225,14 +223,7 @@ __device__ __forceinline__ bool crossing_f(LaneF& L, const DevParams& P, const T
   if ((y0[1] < T(0)) == (y1[1] < T(0))) return false;
   const double rc = crossing_radius(y0, f0, y1, f1, h);
   const double g = (rc >= static_cast<double>(L.r_isco)) ? disk_g(rc, L.cd.a, L.cd.xi) : 0.0;
-  const bool rec_ok = L.ncross < P.max_crossings;
-  const bool s0 = rec_ok && L.ncross == 0, s1 = rec_ok && L.ncross == 1, s2 = rec_ok && L.ncross == 2;
-  L.cr0 = s0 ? rc : L.cr0;
-  L.cg0 = s0 ? g : L.cg0;
-  L.cr1 = s1 ? rc : L.cr1;
-  L.cg1 = s1 ? g : L.cg1;
-  L.cr2 = s2 ? rc : L.cr2;
-  L.cg2 = s2 ? g : L.cg2;
+  if (L.ncross < P.max_crossings) store_crossing(P.out + L.gid, L.ncross, rc, g);
   if (L.ncross < 255) L.ncross++;
   if (P.disk_enabled && rc >= static_cast<double>(L.r_isco) && rc <= P.disk_r_out) {
     const double* lut = P.luts + static_cast<long long>(P.cams[L.cam].lut_index) * P.lut_n;
@@ -264,9 +255,6 @@ __device__ __forceinline__ void write_record_f(const LaneF& L, const DevParams& 
   double2* d2 = reinterpret_cast<double2*>(rec);
   d2[0] = make_double2(static_cast<double>(L.I) - static_cast<double>(L.Ic), yd[0]);
   d2[1] = make_double2(yd[1], yd[2]);
-  d2[2] = make_double2(L.cr0, L.cr1);
-  d2[3] = make_double2(L.cr2, L.cg0);
-  d2[4] = make_double2(L.cg1, L.cg2);
   d2[5] = make_double2(static_cast<double>(L.tau) - static_cast<double>(L.tauc),
                        __builtin_fabs(__builtin_fma(yd[3], yd[3], -R)) / (rs * rs));
   const double dc = __builtin_fabs(__builtin_fma(yd[4], yd[4], -U)) /
