@@ -15,7 +15,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libbhk.so")
 
-SOURCES = ["trace.hip", "trace_fp32.hip", "surrogate.hip", "sur_infer.hip", "render_launch.hip", "host_tables.cpp"]
+SOURCES = ["trace.hip", "trace_fp32.hip", "surrogate.hip", "sur_infer.hip", "render_launch.hip", "bridge.hip",
+           "host_tables.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -43,6 +43,7 @@ struct Lane {
   long long gid;
   int cam;
   int vb;  // volume of a volume set (multi-volume launches)
+  int npts;  // path points written (path recording)
   int steps, attempts, ncross, nsamp;
   bool last_rej;
 };
@@ -137,8 +138,22 @@ __device__ __forceinline__ void write_record(const Lane& L, const DevParams& P, 
   i4[31] = L.nsamp;
 }
 
+// Appends the lane's current position to its polyline (path recording).
+__device__ __forceinline__ void path_point(const DevParams& P, Lane& L) {
+  if (L.npts >= P.path_max) return;
+  const double r = L.y[0], u = L.y[1];
+  const double st = __builtin_sqrt(fmax(__builtin_fma(-u, u, 1.0), 0.0));
+  double sp, cp;
+  sincos(L.y[2], &sp, &cp);
+  float* q = P.path + (L.gid * static_cast<long long>(P.path_max) + L.npts) * 3;
+  q[0] = static_cast<float>(r * st * cp);
+  q[1] = static_cast<float>(r * st * sp);
+  q[2] = static_cast<float>(r * u);
+  L.npts++;
+}
+
 // One RK4 step; returns the terminal status (0 = continue).
-template <bool VOL, bool MV>
+template <bool VOL, bool MV, bool PATH>
 __device__ __forceinline__ int step_rk4(Lane& L, const DevParams& P) {
   if (L.steps >= P.max_steps) return BH_HIT_MAX_STEPS;
   const double h = P.h_fixed;
@@ -155,12 +170,13 @@ __device__ __forceinline__ int step_rk4(Lane& L, const DevParams& P) {
     L.y[i] = yn[i];
     L.k1[i] = fn[i];
   }
+  if (PATH) path_point(P, L);
   if (hit) return BH_HIT_DISK;
   return terminate_check(L, P);
 }
 
 // One DP45 attempt; returns the terminal status (0 = continue).
-template <bool VOL, bool MV>
+template <bool VOL, bool MV, bool PATH>
 __device__ __forceinline__ int step_dp45(Lane& L, const DevParams& P) {
   if (L.steps >= P.max_steps || L.attempts >= P.max_attempts) return BH_HIT_MAX_STEPS;
   double cap = P.h_max;
@@ -182,6 +198,7 @@ __device__ __forceinline__ int step_dp45(Lane& L, const DevParams& P) {
     }
     L.h = L.h * fac;
     L.last_rej = false;
+    if (PATH) path_point(P, L);
     if (hit) return BH_HIT_DISK;
     return terminate_check(L, P);
   }
@@ -193,7 +210,7 @@ __device__ __forceinline__ int step_dp45(Lane& L, const DevParams& P) {
 // Two workgroups per CU: the volume variants would otherwise take a few
 // registers past the 256 that two waves per SIMD allow (the handful of
 // spilled values cost less than the lost latency hiding).
-template <int INTEG, bool VOL, bool MV>
+template <int INTEG, bool VOL, bool MV, bool PATH>
 __global__ void __launch_bounds__(kBlock, 2) k_trace_fp64(const DevParams* __restrict__ Pp) {
   const DevParams& P = *Pp;
   Lane L;
@@ -205,7 +222,12 @@ __global__ void __launch_bounds__(kBlock, 2) k_trace_fp64(const DevParams* __res
     const long long id = refill_id(need, P.head);
     if (need) {
       if (id < n_queue) {
-        if (!lane_setup<VOL>(L, P, id)) L.gid = -1;
+        if (!lane_setup<VOL>(L, P, id)) {
+          L.gid = -1;
+        } else if (PATH) {
+          L.npts = 0;
+          path_point(P, L);
+        }
       } else {
         done = true;
       }
@@ -213,9 +235,10 @@ __global__ void __launch_bounds__(kBlock, 2) k_trace_fp64(const DevParams* __res
     if (__ballot(!done) == 0ull) break;
     if (L.gid >= 0) {
       for (int b = 0; b < P.batch; ++b) {
-        const int st = (INTEG == BH_INTEGRATOR_DP45) ? step_dp45<VOL, MV>(L, P) : step_rk4<VOL, MV>(L, P);
+        const int st = (INTEG == BH_INTEGRATOR_DP45) ? step_dp45<VOL, MV, PATH>(L, P) : step_rk4<VOL, MV, PATH>(L, P);
         if (st) {
           write_record(L, P, st);
+          if (PATH) P.path_count[L.gid] = L.npts;
           L.gid = -1;
           break;
         }
@@ -246,16 +269,17 @@ int fail(const std::string& msg) {
 struct KernelKey {
   int integ, prec;
   bool vol, multi;
+  bool path = false;  // polyline recording (DP45, FP64, no volume only)
 };
 
-template <int I, bool V, bool M>
+template <int I, bool V, bool M, bool PT = false>
 int max_blocks_per_cu() {
   static int cached[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
   if (cached[dev] > 0) return cached[dev];
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_fp64<I, V, M>, kBlock, 0) != hipSuccess) return -1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_fp64<I, V, M, PT>, kBlock, 0) != hipSuccess) return -1;
   cached[dev] = std::max(1, nb);
   return cached[dev];
 }
@@ -273,6 +297,7 @@ int cu_count() {
 
 int occupancy_limit(const KernelKey& k) {
   if (k.prec == BH_PRECISION_FP32) return occupancy_limit_fp32(k.integ, k.vol);
+  if (k.path) return max_blocks_per_cu<BH_INTEGRATOR_DP45, false, false, true>();
   if (k.integ == BH_INTEGRATOR_DP45) {
     if (k.multi) return max_blocks_per_cu<BH_INTEGRATOR_DP45, true, true>();
     return k.vol ? max_blocks_per_cu<BH_INTEGRATOR_DP45, true, false>()
@@ -285,8 +310,10 @@ int occupancy_limit(const KernelKey& k) {
 hipError_t launch_kernel(const KernelKey& k, const DevParams* dp, hipStream_t s, int grid) {
   if (k.prec == BH_PRECISION_FP32) return launch_fp32(k.integ, k.vol, dp, s, grid);
 #define BHK_LAUNCH(I, V, M) \
-  hipLaunchKernelGGL((k_trace_fp64<I, V, M>), dim3(grid), dim3(kBlock), 0, s, dp)
-  if (k.integ == BH_INTEGRATOR_DP45) {
+  hipLaunchKernelGGL((k_trace_fp64<I, V, M, false>), dim3(grid), dim3(kBlock), 0, s, dp)
+  if (k.path) {
+    hipLaunchKernelGGL((k_trace_fp64<BH_INTEGRATOR_DP45, false, false, true>), dim3(grid), dim3(kBlock), 0, s, dp);
+  } else if (k.integ == BH_INTEGRATOR_DP45) {
     if (k.multi) BHK_LAUNCH(BH_INTEGRATOR_DP45, true, true);
     else if (k.vol) BHK_LAUNCH(BH_INTEGRATOR_DP45, true, false);
     else BHK_LAUNCH(BH_INTEGRATOR_DP45, false, false);
@@ -623,6 +650,23 @@ int trace_launch(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
   if (build_launch(cams, n_cams, opts, disk_luts, disk_lut_n, vol, vs, rays, n_rays, ids, n_ids_dev, max_ids, out,
                    compact, who, &im) != 0)
     return -1;
+  return launch_on_stream(im, stream, who);
+}
+
+int trace_paths(const BH_Camera& cam, const BH_TraceOptions& opts, float* d_path, int* d_count, int path_max,
+                BH_PixelRecord* out, void* stream, const char* who) {
+  if (opts.integrator != BH_INTEGRATOR_DP45 || opts.precision != BH_PRECISION_FP64 || opts.volume_enabled ||
+      opts.disk_enabled)
+    return fail(std::string(who) + ": path recording runs DP45 FP64 without disk or volume");
+  if (!d_path || !d_count || path_max < 1) return fail(std::string(who) + ": invalid path buffers");
+  LaunchImage im;
+  if (build_launch(&cam, 1, &opts, nullptr, 0, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 0, out, false, who,
+                   &im) != 0)
+    return -1;
+  im.P.path = d_path;
+  im.P.path_count = d_count;
+  im.P.path_max = path_max;
+  im.key.path = true;
   return launch_on_stream(im, stream, who);
 }
 

@@ -37,6 +37,12 @@ struct DevParams {
   int compact_ids;  // id list: record of entry q -> out[q] instead of out[id]
   const VolBucket* buckets;  // volume set (camera k samples buckets[k.vol_index]); NULL: P.vol
   const PinholeDev* pin;     // pinhole camera (render launch): camera 0's pixels start from it
+  // Path recording (polyline output): the start point and the end point of
+  // every accepted step, Cartesian (r sin th cos phi, r sin th sin phi, r cos th),
+  // at most path_max points per ray, path[(rec * path_max + k) * 3 + c].
+  float* path;
+  int* path_count;
+  int path_max, pad_path;
   // Ray sources other than every pixel of the cameras: an explicit ray list
   // (rays, n_total entries, record i -> out[i]) or a list of pixel ids
   // (ids, count read from count_dev, record -> out[id]).
@@ -163,6 +169,12 @@ int make_cam_devs(const BH_Camera* cams, int n_cams, double horizon_factor, cons
 // edge) on `stream` (trace.hip).
 int trace_pinhole(const BH_Camera& cam, const BH_TraceOptions& opts, const double* disk_lut, int lut_n,
                   const PinholeDev& pin, double r_isco, BH_PixelRecord* out, void* stream, const char* who);
+
+// Host: trace one camera recording every ray's polyline (start point and the
+// end of each accepted step, at most path_max points, counts per ray) into
+// device buffers; DP45 FP64 without disk or volume (trace.hip).
+int trace_paths(const BH_Camera& cam, const BH_TraceOptions& opts, float* d_path, int* d_count, int path_max,
+                BH_PixelRecord* out, void* stream, const char* who);
 
 // FP32 precision form (trace_fp32.hip): launch on `grid` workgroups, and
 // the workgroups per CU its occupancy allows.

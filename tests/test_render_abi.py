@@ -136,3 +136,30 @@ def test_texture_handles_and_variant_selection():
     assert lib().bh_lut_texture_handle(9) == 0 and lib().bh_lut_texture_handle(-1) == 0
     lib().bh_upload_lut_textures(0, 0, 0, 0, 0, 0, 0, 0)
     assert lib().bh_select_kernel_variant() == 2
+
+
+def test_bridge_probes_match_reference_struct_sizes():
+    """sizeof probes: BhbSourceParams = char[64] + 7 doubles = 120 bytes and
+    BhbDiskParams = 5 doubles = 40 bytes (blender_bridge.h:48-57, :180-186);
+    versions 1.0; snake_case aliases exported."""
+    L = lib()
+    assert L.bhbSizeofSourceParams() == 120 and L.bhbSizeofDiskParams() == 40
+    assert L.bhb_sizeof_source_params() == 120 and L.bhb_sizeof_disk_params() == 40
+    assert (L.bhbVersionMajor(), L.bhbVersionMinor(), L.bhbHasCuda(), L.bhbHasBoost()) == (1, 0, 1, 0)
+    assert (L.bhb_version_major(), L.bhb_has_cuda()) == (1, 1)
+
+
+def test_bridge_rejects_invalid_arguments_without_gpu():
+    L = lib()
+    f = L.bhbCudaTraceGeodesics
+    f.argtypes = [C.c_double] * 5 + [C.c_int] + [C.c_double] * 2 + [C.c_int, C.c_int, C.c_double, C.c_void_p,
+                                                                   C.c_void_p]
+    buf = (C.c_float * 300)()
+    cnt = (C.c_int * 4)()
+    assert f(1.0, 50.0, 1.0, -5, 5, 2, -5, 5, 2, 10, 0.1, buf, cnt) == -1 and b"spin" in L.bh_last_error()
+    assert f(0.5, 1.1, 1.0, -5, 5, 2, -5, 5, 2, 10, 0.1, buf, cnt) == -1 and b"horizon" in L.bh_last_error()
+    assert f(0.5, 50.0, 0.0, -5, 5, 2, -5, 5, 2, 10, 0.1, buf, cnt) == -1 and b"inclination" in L.bh_last_error()
+    assert f(0.5, 50.0, 1.0, -5, 5, 2, -5, 5, 2, 10, 0.1, None, cnt) == -1
+    g = L.bhbCudaRenderLensingMap
+    g.argtypes = [C.c_double] * 3 + [C.c_int, C.c_int, C.c_void_p]
+    assert g(0.5, 50.0, 1.0, 0, 4, buf) == -1
