@@ -93,6 +93,25 @@ def lib():
                                 C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p]
     L.bh_trace_rays.restype = C.c_int
     L.bh_last_kernel_ms.restype = C.c_float
+    # renderer backend (bh_backend.h)
+    L.bhCudaInit.argtypes = [C.c_uint, C.c_int, C.c_int]
+    L.bhCudaInit.restype = C.c_void_p
+    L.bhCudaRenderFrame.argtypes = [C.c_void_p, C.POINTER(abi.BH_LaunchParams)]
+    L.bhCudaResize.argtypes = [C.c_void_p, C.c_uint, C.c_int, C.c_int]
+    L.bhCudaRegisterLut.argtypes = [C.c_void_p, C.c_int, C.c_uint, C.c_uint]
+    L.bhCudaRegisterBackgroundTexture.argtypes = [C.c_void_p, C.c_uint, C.c_uint]
+    L.bhCudaSetVariant.argtypes = [C.c_void_p, C.c_int]
+    L.bhCudaGetVariant.argtypes = [C.c_void_p]
+    L.bhCudaSynchronize.argtypes = [C.c_void_p]
+    for name in ("bhCudaRenderFrame", "bhCudaResize", "bhCudaRegisterLut", "bhCudaRegisterBackgroundTexture",
+                 "bhCudaSetVariant", "bhCudaGetVariant", "bhCudaSynchronize"):
+        getattr(L, name).restype = C.c_int
+    L.bhCudaShutdown.argtypes = [C.c_void_p]
+    L.bhCudaShutdown.restype = None
+    L.bhCudaFramebuffer.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.bhCudaFramebuffer.restype = C.c_void_p
+    L.bhCudaGlTexture.argtypes = [C.c_void_p, C.c_int]
+    L.bhCudaGlTexture.restype = C.c_uint
     for name in (
         "bh_version", "bh_trace_options_default", "bh_disk_lut_build",
         "bh_volume_generate_torus", "bh_trace_image_plane",

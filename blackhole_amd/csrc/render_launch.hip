@@ -264,3 +264,106 @@ unsigned long long bh_lut_texture_handle(int slot) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Renderer backend (include/bh_backend.h)
+// ---------------------------------------------------------------------------
+
+#include "../../include/bh_backend.h"
+
+struct BhCudaBackend {
+  float4* fb = nullptr;
+  int width = 0, height = 0;
+  int variant = BH_KERNEL_FP16_STORAGE;
+  hipStream_t stream = nullptr;
+  unsigned int gl_fb = 0;
+  unsigned int gl_lut[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+extern "C" {
+
+BH_CudaBackend* bhCudaInit(unsigned int glTexture, int width, int height) {
+  if (width <= 0 || height <= 0) {
+    fail("bhCudaInit: empty framebuffer");
+    return nullptr;
+  }
+  BH_CudaBackend* b = new BH_CudaBackend();
+  b->gl_fb = glTexture;
+  b->variant = bh_select_kernel_variant();
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&b->fb, static_cast<size_t>(width) * height * sizeof(float4)) != hipSuccess) {
+    fail("bhCudaInit: device allocation failed");
+    bhCudaShutdown(b);
+    return nullptr;
+  }
+  b->width = width;
+  b->height = height;
+  return b;
+}
+
+int bhCudaRenderFrame(BH_CudaBackend* b, const struct BH_LaunchParams* params) {
+  if (!b || !params) return fail("bhCudaRenderFrame: NULL argument");
+  if (params->width != b->width || params->height != b->height)
+    return fail("bhCudaRenderFrame: params size differs from the framebuffer (bhCudaResize first)");
+  return bh_launch_geodesic_kernel(b->fb, params, b->variant, b->stream);
+}
+
+int bhCudaResize(BH_CudaBackend* b, unsigned int glTexture, int width, int height) {
+  if (!b || width <= 0 || height <= 0) return fail("bhCudaResize: invalid arguments");
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return fail("bhCudaResize: device error");
+  if (b->fb) (void)hipFree(b->fb);
+  b->fb = nullptr;
+  if (hipMalloc(&b->fb, static_cast<size_t>(width) * height * sizeof(float4)) != hipSuccess)
+    return fail("bhCudaResize: device allocation failed");
+  b->width = width;
+  b->height = height;
+  b->gl_fb = glTexture;
+  return 0;
+}
+
+int bhCudaRegisterLut(BH_CudaBackend* b, int slot, unsigned int glTexture, unsigned int /*target*/) {
+  if (!b || slot < 0 || slot > 7) return fail("bhCudaRegisterLut: invalid arguments");
+  b->gl_lut[slot] = glTexture;
+  return 0;
+}
+
+int bhCudaRegisterBackgroundTexture(BH_CudaBackend* b, unsigned int glTexture, unsigned int /*target*/) {
+  if (!b) return fail("bhCudaRegisterBackgroundTexture: NULL backend");
+  b->gl_lut[8] = glTexture;
+  return 0;
+}
+
+int bhCudaSetVariant(BH_CudaBackend* b, int variant) {
+  if (!b) return -1;
+  b->variant = resolve_variant(variant);
+  return b->variant;
+}
+
+int bhCudaGetVariant(BH_CudaBackend* b) { return b ? b->variant : -1; }
+
+void bhCudaShutdown(BH_CudaBackend* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->fb) (void)hipFree(b->fb);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+void* bhCudaFramebuffer(BH_CudaBackend* b, int* width, int* height) {
+  if (!b) return nullptr;
+  if (width) *width = b->width;
+  if (height) *height = b->height;
+  return b->fb;
+}
+
+int bhCudaSynchronize(BH_CudaBackend* b) {
+  if (!b) return fail("bhCudaSynchronize: NULL backend");
+  return hipStreamSynchronize(b->stream) == hipSuccess ? 0 : fail("bhCudaSynchronize: device error");
+}
+
+unsigned int bhCudaGlTexture(BH_CudaBackend* b, int slot) {
+  if (!b || slot < -1 || slot > 8) return 0u;
+  return slot < 0 ? b->gl_fb : b->gl_lut[slot];
+}
+
+}  // extern "C"

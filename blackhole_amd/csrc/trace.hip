@@ -811,6 +811,26 @@ int bh_trace_rays(const BH_RaySpec* rays, int64_t n, const BH_TraceOptions* opts
                       "bh_trace_rays");
 }
 
+int bh_trace_rays_host(const BH_RaySpec* rays, int64_t n, const BH_TraceOptions* opts, BH_PixelRecord* out) {
+  if (!rays || !out || n <= 0) return fail("bh_trace_rays_host: invalid arguments");
+  BH_RaySpec* d_rays = nullptr;
+  BH_PixelRecord* d_out = nullptr;
+  int rc = 0;
+  if (hipMalloc(&d_rays, n * sizeof(BH_RaySpec)) != hipSuccess || hipMalloc(&d_out, n * sizeof(BH_PixelRecord)) != hipSuccess) {
+    rc = fail("bh_trace_rays_host: device allocation failed");
+  } else if (hipMemcpy(d_rays, rays, n * sizeof(BH_RaySpec), hipMemcpyHostToDevice) != hipSuccess) {
+    rc = fail("bh_trace_rays_host: upload failed");
+  } else if (bh_trace_rays(d_rays, n, opts, nullptr, d_out, nullptr) != 0) {
+    rc = -1;
+  } else if (hipStreamSynchronize(nullptr) != hipSuccess ||
+             hipMemcpy(out, d_out, n * sizeof(BH_PixelRecord), hipMemcpyDeviceToHost) != hipSuccess) {
+    rc = fail("bh_trace_rays_host: trace or copy failed");
+  }
+  if (d_rays) (void)hipFree(d_rays);
+  if (d_out) (void)hipFree(d_out);
+  return rc;
+}
+
 float bh_last_kernel_ms(void) {
   if (!g_have_events) return -1.0f;
   float ms = -1.0f;

@@ -324,6 +324,13 @@ typedef struct BH_TuneResult {
 } BH_TuneResult;
 int bh_frame_autotune(BH_Frame* f, int sample_stride, BH_TuneResult* best, void* stream);
 
+/* bh_trace_rays with host buffers: copies the n ray specs to the device,
+ * traces them and copies the records back (synchronous; device buffers are
+ * allocated for the call).  For callers that hold host memory only (FFI
+ * bindings, the physics C++ interface). */
+int bh_trace_rays_host(const BH_RaySpec* rays, int64_t n, const BH_TraceOptions* opts,
+                       BH_PixelRecord* out);
+
 /* Elapsed time of the last bh_trace_image_plane kernel on this thread, from
  * HIP events recorded around the launch on its stream (ms; -1 if none).  The
  * caller must have synchronised the stream. */

@@ -1,0 +1,85 @@
+// Prints values of the physics C++ interface (include/bh_physics.hpp) for
+// tests/test_physics_api.py, in geometric units: mass chosen so that
+// G M / c^2 = 1 cm.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "bh_physics.hpp"
+
+int main(int argc, char** argv) {
+  const double mass = physics::C2 / physics::G;  // M = 1 cm
+  const char* what = argc > 1 ? argv[1] : "";
+  if (!std::strcmp(what, "closed")) {
+    for (double s : {0.0, 0.3, 0.9, 0.998}) {
+      std::printf("%.17g %.17g %.17g %.17g %.17g %.17g\n", s, physics::kerrOuterHorizon(mass, s),
+                  physics::kerrIscoRadius(mass, s, true), physics::kerrIscoRadius(mass, s, false),
+                  physics::kerrPhotonOrbitPrograde(mass, s), physics::kerrPhotonOrbitRetrograde(mass, s));
+    }
+  } else if (!std::strcmp(what, "potentials")) {
+    // a xi eta r theta -> R R' Theta Theta'
+    double a, xi, eta, r, th;
+    while (std::scanf("%lf %lf %lf %lf %lf", &a, &xi, &eta, &r, &th) == 5) {
+      const physics::KerrGeodesicConsts c{1.0, xi, eta};
+      const physics::KerrPotentials p = physics::kerrPotentials(r, th, mass, a, c);
+      std::printf("%.17g %.17g %.17g %.17g\n", p.rPot, p.dRdr, p.thetaPot, p.dThetadtheta);
+    }
+  } else if (!std::strcmp(what, "turning")) {
+    // equatorial a = 0.9, b = 6 inbound from r = 12: smallest r reached, then outbound?
+    physics::Kerr k(mass, 0.9);
+    physics::KerrGeodesicState s = physics::kerrEquatorialState(12.0, 0.0, -1.0);
+    const physics::KerrGeodesicConsts c = physics::kerrEquatorialConsts(6.0);
+    double rmin = s.r;
+    int turned = 0;
+    for (int i = 0; i < 40000; ++i) {
+      s = k.stepMino(s, c, 2e-4);
+      rmin = std::fmin(rmin, s.r);
+      if (s.signR > 0 && s.r > 6.0) { turned = 1; break; }
+    }
+    std::printf("%.17g %d\n", rmin, turned);
+  } else if (!std::strcmp(what, "order")) {
+    // RK4 order of kerrStepMino: error of N steps of h against 2N steps of h/2 and 4N of h/4
+    physics::Kerr k(mass, 0.7);
+    const physics::KerrGeodesicConsts c{1.0, 2.5, 9.0};
+    auto run = [&](double h, int n) {
+      physics::KerrGeodesicState s{20.0, 1.2, 0.0, 0.0, -1.0, 1.0};
+      for (int i = 0; i < n; ++i) s = k.stepMino(s, c, h);
+      return s.r;
+    };
+    const double r1 = run(4e-4, 50), r2 = run(2e-4, 100), r3 = run(1e-4, 200);
+    std::printf("%.17g %.17g\n", std::fabs(r1 - r2), std::fabs(r2 - r3));
+  } else if (!std::strcmp(what, "verified")) {
+    auto f = [](const verified::StateVector& y) {
+      return verified::StateVector(y.x0, y.x1, y.x2, y.x3, y.v0, y.v1, y.v2, y.v3);  // y' = y
+    };
+    verified::StateVector y0(1, 1, 1, 1, 1, 1, 1, 1);
+    const double e1 = std::fabs(verified::rk4_step(f, 0.1, y0).x1 - std::exp(0.1));
+    const auto d1 = verified::dp45_step(f, 0.2, y0, 0.0, 1.0);
+    const auto d2 = verified::dp45_step(f, 0.1, y0, 0.0, 1.0);
+    std::printf("%.17g %.17g %.17g %.17g %.17g\n", e1, std::fabs(d1.first.x1 - std::exp(0.2)), d1.second, d2.second,
+                verified::dp45_next_step(1.0, 1.0, false));
+  } else if (!std::strcmp(what, "raytrace")) {
+    // KerrRaytracer on the GPU: equatorial rays from r = 80 M with b in {4, 5, 6, 7}, a = 0.5
+    physics::KerrRaytracer rt(mass, 0.5);
+    rt.setEscapeRadius(200.0);
+    rt.setMaxSteps(20000);
+    std::vector<std::pair<physics::KerrGeodesicState, physics::KerrGeodesicConsts>> rays;
+    for (double b : {4.0, 5.0, 6.0, 7.0})
+      rays.push_back({physics::kerrEquatorialState(80.0, 0.0, -1.0), physics::kerrEquatorialConsts(b)});
+    for (const physics::RayTraceResult& r : rt.traceBatch(rays))
+      std::printf("%d %d %.17g %.17g %.17g\n", static_cast<int>(r.status), r.stepsTaken, r.finalPosition[0],
+                  r.finalPosition[1], r.finalPosition[2]);
+  } else if (!std::strcmp(what, "rayspec")) {
+    double a, alpha, beta, th, r;
+    while (std::scanf("%lf %lf %lf %lf %lf", &a, &alpha, &beta, &th, &r) == 5) {
+      const double xi = -alpha * std::sin(th);
+      const double eta = beta * beta + (alpha * alpha - a * a) * std::cos(th) * std::cos(th);
+      const physics::KerrGeodesicConsts c{1.0, xi, eta};
+      const physics::KerrGeodesicState s{r, th, 0.0, 0.0, -1.0, beta >= 0 ? -1.0 : 1.0};
+      BH_RaySpec sp;
+      const bool ok = physics::raySpecFor(s, c, 1.0, a, &sp);
+      std::printf("%d %.17g %.17g %.17g %.17g %.17g\n", ok ? 1 : 0, sp.alpha, sp.beta, sp.spin, sp.theta_obs, sp.r_obs);
+    }
+  }
+  return 0;
+}

@@ -163,3 +163,18 @@ def test_bridge_rejects_invalid_arguments_without_gpu():
     g = L.bhbCudaRenderLensingMap
     g.argtypes = [C.c_double] * 3 + [C.c_int, C.c_int, C.c_void_p]
     assert g(0.5, 50.0, 1.0, 0, 4, buf) == -1
+
+
+def test_backend_handles_null_and_empty_without_gpu():
+    L = lib()
+    assert L.bhCudaInit(0, 0, 16) is None
+    assert "empty framebuffer" in L.bh_last_error().decode()
+    assert L.bhCudaInit(0, 16, -1) is None
+    L.bhCudaShutdown(None)
+    assert L.bhCudaGetVariant(None) == -1 and L.bhCudaSetVariant(None, 0) == -1
+    assert L.bhCudaRenderFrame(None, None) == -1
+    assert L.bhCudaResize(None, 0, 4, 4) == -1
+    assert L.bhCudaRegisterLut(None, 0, 1, 0) == -1
+    assert L.bhCudaSynchronize(None) == -1
+    assert L.bhCudaFramebuffer(None, None, None) is None
+    assert L.bhCudaGlTexture(None, -1) == 0
