@@ -93,6 +93,9 @@ def lib():
                                 C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p]
     L.bh_trace_rays.restype = C.c_int
     L.bh_last_kernel_ms.restype = C.c_float
+    L.bh_trace_disk_mino.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.BH_TraceOptions), C.c_void_p, C.c_int,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+    L.bh_trace_disk_mino.restype = C.c_int
     # renderer backend (bh_backend.h)
     L.bhCudaInit.argtypes = [C.c_uint, C.c_int, C.c_int]
     L.bhCudaInit.restype = C.c_void_p

@@ -19,6 +19,7 @@ FLAG_FP64_ISLAND = 0x01
 FLAG_FALLBACK = 0x02
 FLAG_AUDITED = 0x04
 FLAG_SURROGATE = 0x08
+FLAG_REFINED = 0x10
 
 
 class BH_Camera(C.Structure):

@@ -1,0 +1,426 @@
+// mino.hpp - equatorial crossings of Kerr null geodesics from the Mino-time
+// integrals of motion (host and device).
+//
+// In Mino time l the radial and polar motions separate:
+//   (dr/dl)^2 = R(r) = r^4 + A r^2 + B r + C,
+//       A = a^2 - eta - xi^2,  B = 2 (eta + (xi - a)^2),  C = -a^2 eta,
+//   (du/dl)^2 = U(u) = eta - (eta + xi^2 - a^2) u^2 - a^2 u^4,  u = cos(theta),
+// so a ray traced back from the observer (r_o, u_o) reaches its m-th
+// equatorial crossing (u = 0) after the Mino time
+//   G_m = (F_o or 2K - F_o) / k + m 2K / k
+// (F_o = F(asin(|u_o| / sqrt(w+)) | mpar), K = K(mpar), k = sqrt(eta / w+),
+// mpar = -a^2 w+^2 / eta, w+ the positive root of a^2 w^2 + (eta + xi^2 -
+// a^2) w - eta = 0; F_o when the ray starts toward the equator), and the
+// crossing radius solves  integral from r to r_o of dr / sqrt(R) = G_m  along
+// the radial path (through the turning point r4, the largest root of R, when
+// r4 lies outside the stopping radius).  The polar integrals are Carlson's
+// R_F (B. C. Carlson, "Numerical computation of real or complex elliptic
+// integrals", Numer. Algorithms 10 (1995) 13-26, duplication algorithm); the
+// radial roots come from Ferrari's resolvent cubic; the radial integral is
+// taken in x = 1/r -- with x = x4 - s^2 when the path turns, which makes the
+// integrand 2 / sqrt(S(x4 - s^2)) (S = x^4 R(1/x) / (x4 - x), a cubic) smooth
+// through the turning point -- by adaptive Gauss-Kronrod 7-15 quadrature
+// (the QUADPACK rule), and the crossing by a bracketed Newton iteration on
+// T(w) - G_m, T the Mino time along the path variable w.
+//
+// The same geodesic equations are what the step integrators of the library
+// integrate (kerr_device.hpp, second-order form); here they are solved in
+// quadrature instead, to a relative tolerance far below the integrators'
+// crossing interpolation.  Rays the quadrature cannot resolve (the integrand
+// nearly singular, as close to the critical curve) report failure.
+#pragma once
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MINO_HD __host__ __device__ inline
+#else
+#include <cmath>
+#define MINO_HD inline
+#endif
+
+namespace mino {
+
+#if !defined(__HIPCC__)
+using std::acos;
+using std::cbrt;
+using std::cos;
+using std::fabs;
+using std::fmax;
+using std::fmin;
+using std::sqrt;
+#endif
+
+// Carlson's R_F(x, y, z) for x, y, z >= 0, at most one zero: duplication until
+// the arguments agree to 2.5e-3, then the fifth-order series
+// (1 - E2/10 + E3/14 + E2^2/24 - 3 E2 E3 / 44) / sqrt(A).
+MINO_HD double carlson_rf(double x, double y, double z) {
+  double A = (x + y + z) * (1.0 / 3.0);
+  for (int it = 0; it < 60; ++it) {
+    const double dx = 1.0 - x / A, dy = 1.0 - y / A, dz = 1.0 - z / A;
+    if (fmax(fabs(dx), fmax(fabs(dy), fabs(dz))) < 2.5e-3) {
+      const double X = dx, Y = dy, Z = -(dx + dy);
+      const double E2 = X * Y - Z * Z, E3 = X * Y * Z;
+      return (1.0 - E2 / 10.0 + E3 / 14.0 + E2 * E2 / 24.0 - 3.0 * E2 * E3 / 44.0) / sqrt(A);
+    }
+    const double sx = sqrt(x), sy = sqrt(y), sz = sqrt(z);
+    const double lam = sx * (sy + sz) + sy * sz;
+    x = 0.25 * (x + lam);
+    y = 0.25 * (y + lam);
+    z = 0.25 * (z + lam);
+    A = (x + y + z) * (1.0 / 3.0);
+  }
+  return 1.0 / sqrt(A);
+}
+
+// Mino times of the polar motion: the m-th equatorial crossing comes at
+// g0 + m dg.  ok = 0 when the ray never reaches the equator (eta <= 0, or an
+// observer in the equatorial plane).
+struct PolarTimes {
+  double g0, dg;
+  int ok;
+};
+
+// u_dir: sign of du/dl at the observer (the tracer's initial v_u = sin(theta_o) beta).
+MINO_HD PolarTimes polar_times(double a, double xi, double eta, double u_o, double u_dir) {
+  PolarTimes t{0.0, 0.0, 0};
+  if (!(eta > 0.0) || u_o == 0.0) return t;
+  const double a2 = a * a;
+  const double b = eta + xi * xi - a2;
+  const double disc = sqrt(b * b + 4.0 * a2 * eta);
+  const double wp = (b >= 0.0) ? 2.0 * eta / (b + disc) : (disc - b) / (2.0 * a2);
+  const double k = sqrt(eta / wp);
+  const double mpar = -a2 * wp * wp / eta;
+  double s = fabs(u_o) / sqrt(wp);
+  if (s > 1.0) s = 1.0;
+  const double s2 = s * s;
+  const double Fo = s * carlson_rf(1.0 - s2, 1.0 - mpar * s2, 1.0);
+  const double K = carlson_rf(0.0, 1.0 - mpar, 1.0);
+  const bool toward = (u_o > 0.0) ? (u_dir < 0.0) : (u_dir > 0.0);
+  t.g0 = (toward ? Fo : 2.0 * K - Fo) / k;
+  t.dg = 2.0 * K / k;
+  t.ok = 1;
+  return t;
+}
+
+// Largest real root of R when R has real roots r3 <= r4 (has34 = 1), from the
+// largest root y0 of the resolvent 8y^3 + 8A y^2 + (2A^2 - 8C) y - B^2 = 0:
+// with z = sqrt(y0 / 2), r3,4 = z -+ sqrt(-A/2 - z^2 - B / (4z)) and
+// r1,2 = -z -+ sqrt(-A/2 - z^2 + B / (4z)).  r4 is polished by Newton on R.
+struct RadialRoots {
+  double r3, r4;
+  int has34;
+};
+
+MINO_HD double quartic_R(double A, double B, double C, double r) { return ((r * r + A) * r + B) * r + C; }
+
+MINO_HD RadialRoots radial_roots(double A, double B, double C) {
+  RadialRoots o{0.0, 0.0, 0};
+  const double c1 = 0.25 * A * A - C;  // y^3 + A y^2 + c1 y - B^2/8
+  const double c0 = -0.125 * B * B;
+  const double P = c1 - A * A / 3.0;
+  const double Q = 2.0 * A * A * A / 27.0 - A * c1 / 3.0 + c0;
+  const double D = 0.25 * Q * Q + P * P * P / 27.0;
+  double t;
+  if (D > 0.0) {
+    const double h = -0.5 * Q;
+    const double u1 = cbrt(h + (h >= 0.0 ? sqrt(D) : -sqrt(D)));
+    t = (u1 != 0.0) ? u1 - P / (3.0 * u1) : 0.0;
+  } else {
+    const double rp = sqrt(fmax(-P / 3.0, 0.0));
+    double c = (rp > 0.0) ? -Q / (2.0 * rp * rp * rp) : 0.0;
+    c = fmin(fmax(c, -1.0), 1.0);
+    t = 2.0 * rp * cos(acos(c) / 3.0);
+  }
+  double y = t - A / 3.0;
+  for (int it = 0; it < 3; ++it) {
+    const double h = ((y + A) * y + c1) * y + c0;
+    const double dh = (3.0 * y + 2.0 * A) * y + c1;
+    if (dh != 0.0) y -= h / dh;
+  }
+  if (!(y > 0.0)) return o;
+  const double z = sqrt(0.5 * y);
+  const double rad = -0.5 * A - z * z - B / (4.0 * z);
+  if (rad < 0.0) return o;
+  const double sr = sqrt(rad);
+  double r4 = z + sr;
+  for (int it = 0; it < 2; ++it) {
+    const double f = quartic_R(A, B, C, r4);
+    const double df = (4.0 * r4 * r4 + 2.0 * A) * r4 + B;
+    if (df > 0.0) r4 -= f / df;
+  }
+  o.r3 = z - sr;
+  o.r4 = r4;
+  o.has34 = 1;
+  return o;
+}
+
+// The radial path of one ray as a function of w >= 0 (Mino time T(w) =
+// integral of f from 0 to w):
+//   turning:  s = s_o - w, x = x4 - s^2, f = 2 / sqrt(S(x)), S the cubic
+//             x^4 R(1/x) / (x4 - x) = -(q3 x^3 + q2 x^2 + q1 x + q0);
+//   plunging: x = x_o + w, f = 1 / sqrt(x^4 R(1/x)).
+struct Path {
+  double A, B, C;
+  double x_o, x4, s_o;
+  double q3, q2, q1, q0;
+  double w_end;  // stopping radius (plunging) or escape radius (turning)
+  int turning;
+};
+
+MINO_HD double path_x(const Path& p, double w) {
+  if (p.turning) {
+    const double s = p.s_o - w;
+    return p.x4 - s * s;
+  }
+  return p.x_o + w;
+}
+
+MINO_HD double path_f(const Path& p, double w) {
+  if (p.turning) {
+    const double s = p.s_o - w;
+    const double x = p.x4 - s * s;
+    const double S = -(((p.q3 * x + p.q2) * x + p.q1) * x + p.q0);
+    return S > 0.0 ? 2.0 / sqrt(S) : 1e300;
+  }
+  const double x = p.x_o + w;
+  const double Rx = 1.0 + x * x * (p.A + x * (p.B + p.C * x));
+  return Rx > 0.0 ? 1.0 / sqrt(Rx) : 1e300;
+}
+
+// One Gauss-Kronrod 7-15 panel (the QUADPACK qk15 rule: Kronrod abscissae
+// xgk, Kronrod weights wgk, Gauss weights wg of the odd abscissae and the
+// centre): returns the Kronrod estimate, *err = |K15 - G7|.
+MINO_HD double gk15(const Path& p, double a, double b, double* err) {
+  constexpr double xgk[7] = {0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+                             0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+                             0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+                             0.207784955007898467600689403773245};
+  constexpr double wgk[8] = {0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+                             0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+                             0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+                             0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+  constexpr double wg[4] = {0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+                            0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+  const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+  const double fc = path_f(p, c);
+  double rk = wgk[7] * fc, rg = wg[3] * fc;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const double dx = h * xgk[j];
+    const double f = path_f(p, c - dx) + path_f(p, c + dx);
+    rk += wgk[j] * f;
+    if (j & 1) rg += wg[j >> 1] * f;
+  }
+  *err = fabs((rk - rg) * h);
+  return rk * h;
+}
+
+// Adaptive GK15 over [a, b] (a <= b), marching from a with step control
+// (no panel list, so it lives in registers): a panel is accepted when
+// |K15 - G7| <= tol |K15| (the relative error of the sum is then below tol),
+// the next step doubles; a rejected panel is halved.  *ok = 0 when it needed
+// more than kMaxPanels panel evaluations.
+constexpr int kMaxPanels = 48;
+
+MINO_HD double integrate(const Path& p, double a, double b, double tol, int* ok) {
+  double sum = 0.0, x = a, h = b - a;
+  const double hmin = 1e-13 * (b - a);
+  for (int it = 0; it < kMaxPanels; ++it) {
+    if (!(x < b)) {
+      *ok = 1;
+      return sum;
+    }
+    if (h > b - x) h = b - x;
+    double e;
+    const double v = gk15(p, x, x + h, &e);
+    if (e <= tol * fabs(v) || h <= hmin) {
+      sum += v;
+      x += h;
+      h *= 2.0;
+    } else {
+      h *= 0.5;
+    }
+  }
+  *ok = !(x < b);
+  return sum;
+}
+
+// Builds the radial path of a ray from r_o inward: turning when R has a root
+// r4 above r_stop (then it ends at r_escape on the way out), plunging
+// otherwise (ends at r_stop).  Returns 0 when the turning point is too close
+// to the observer or the roots degenerate.
+MINO_HD int make_path(double a, double xi, double eta, double r_o, double r_stop, double r_escape, Path* p) {
+  p->A = a * a - eta - xi * xi;
+  const double dx = xi - a;
+  p->B = 2.0 * (eta + dx * dx);
+  p->C = -a * a * eta;
+  p->x_o = 1.0 / r_o;
+  const RadialRoots rr = radial_roots(p->A, p->B, p->C);
+  p->turning = rr.has34 && rr.r4 > r_stop;
+  if (p->turning) {
+    if (!(rr.r4 < r_o)) return 0;
+    const double x4 = 1.0 / rr.r4;
+    p->x4 = x4;
+    // x^4 R(1/x) = C x^4 + B x^3 + A x^2 + 0 x + 1, divided by (x - x4)
+    p->q3 = p->C;
+    p->q2 = p->B + x4 * p->q3;
+    p->q1 = p->A + x4 * p->q2;
+    p->q0 = x4 * p->q1;
+    p->s_o = sqrt(x4 - p->x_o);
+    const double x_esc = 1.0 / r_escape;
+    p->w_end = p->s_o + (x4 > x_esc ? sqrt(x4 - x_esc) : 0.0);
+  } else {
+    p->x4 = 0.0;
+    p->q3 = p->q2 = p->q1 = p->q0 = 0.0;
+    p->s_o = 0.0;
+    p->w_end = 1.0 / r_stop - p->x_o;
+  }
+  return 1;
+}
+
+// Solves T(w) = G for w in [w_lo, w_hi] with T(w_lo) = T_lo <= G <= T_hi =
+// T(w_hi), starting from w0 (clamped into the bracket): Newton steps
+// w += (G - T) / f(w), each T update an adaptive integral from the previous
+// iterate, bisection whenever a step leaves the bracket.  Returns the number
+// of iterations, or -1 on failure; *w_out the solution.
+MINO_HD int solve_crossing(const Path& p, double G, double w_lo, double T_lo, double w_hi, double T_hi, double w0,
+                           double tol, double* w_out) {
+  double lo = w_lo, Tlo = T_lo, hi = w_hi, Thi = T_hi;
+  double w = w0;
+  if (!(w > lo && w < hi)) w = lo + (hi - lo) * ((G - Tlo) / (Thi - Tlo));
+  int ok = 1;
+  // T at the starting point from the nearer end of the bracket
+  double T;
+  if (w - lo <= hi - w) {
+    T = Tlo + integrate(p, lo, w, tol, &ok);
+  } else {
+    T = Thi - integrate(p, w, hi, tol, &ok);
+  }
+  if (!ok) return -1;
+  for (int it = 1; it <= 40; ++it) {
+    const double dT = G - T;
+    if (fabs(dT) <= 1e-12 * G + 1e-300) {
+      *w_out = w;
+      return it;
+    }
+    if (dT > 0.0) {
+      lo = w;
+      Tlo = T;
+    } else {
+      hi = w;
+      Thi = T;
+    }
+    const double f = path_f(p, w);
+    double wn = w + dT / f;
+    if (!(wn > lo && wn < hi)) wn = 0.5 * (lo + hi);
+    const double inc = (wn > w) ? integrate(p, w, wn, tol, &ok) : -integrate(p, wn, w, tol, &ok);
+    if (!ok) return -1;
+    T += inc;
+    w = wn;
+    if (hi - lo <= 1e-15 * (fabs(hi) + 1e-300)) {
+      *w_out = w;
+      return it;
+    }
+  }
+  return -1;
+}
+
+// Crossing radius -> path variable w on the leg where it lies (inbound when
+// w < s_o for a turning path); -1 when r is not on the path.
+MINO_HD double w_of_r(const Path& p, double r, bool outbound) {
+  const double x = 1.0 / r;
+  if (p.turning) {
+    if (!(x <= p.x4)) return -1.0;
+    const double s = sqrt(p.x4 - x);
+    return outbound ? p.s_o + s : p.s_o - s;
+  }
+  return x - p.x_o;
+}
+
+// Disk classes as the tracer's records number them.
+constexpr int kHorizon = 1, kDisk = 2, kEscaped = 3;
+constexpr int kMaxCrossings = 16;
+
+// The crossing sequence of one ray traced back from (r_o, u_o) with initial
+// du/dl sign u_dir, up to the first crossing inside [r_in, r_out] (an opaque
+// disk), the stopping radius r_stop or the escape radius r_escape.
+// guess[0 .. n_guess) are starting radii for the first crossings (e.g. a
+// learned estimate); without them each crossing starts from the linear
+// interpolant of its bracket.
+struct RaySolution {
+  double r[3];   // first three crossing radii (0 beyond n)
+  double r_hit;  // radius of the disk crossing (0 without one)
+  int n;         // crossings before the ray ended (the hit included)
+  int hit;       // index of the disk crossing, -1 for none
+  int cls;       // kHorizon, kDisk or kEscaped
+  int iters;     // Newton iterations over all crossings
+  int ok;        // 0: the quadrature or the iteration failed
+};
+
+MINO_HD RaySolution solve_ray(double a, double xi, double eta, double r_o, double u_o, double u_dir,
+                              double r_stop, double r_escape, double r_in, double r_out, const double* guess,
+                              int n_guess, double tol) {
+  RaySolution out{{0.0, 0.0, 0.0}, 0.0, 0, -1, kHorizon, 0, 0};
+  Path p;
+  if (!make_path(a, xi, eta, r_o, r_stop, r_escape, &p)) return out;
+  const int end_cls = p.turning ? kEscaped : kHorizon;
+  const PolarTimes pt = polar_times(a, xi, eta, u_o, u_dir);
+  int ok = 1;
+  double T4 = 0.0, T_total;
+  if (p.turning) {
+    T4 = integrate(p, 0.0, p.s_o, tol, &ok);
+    if (!ok) return out;
+    T_total = T4 + integrate(p, p.s_o, p.w_end, tol, &ok);
+  } else {
+    T_total = integrate(p, 0.0, p.w_end, tol, &ok);
+  }
+  if (!ok) return out;
+  out.cls = end_cls;
+  if (!pt.ok) {
+    out.ok = 1;
+    return out;
+  }
+  double w_prev = 0.0, T_prev = 0.0;
+  for (int m = 0; m < kMaxCrossings; ++m) {
+    const double G = pt.g0 + m * pt.dg;
+    if (!(G < T_total)) {
+      out.ok = 1;
+      return out;
+    }
+    double lo = w_prev, Tlo = T_prev, hi = p.w_end, Thi = T_total;
+    bool outbound = false;
+    if (p.turning) {
+      if (G <= T4) {
+        hi = p.s_o;
+        Thi = T4;
+      } else {
+        outbound = true;
+        if (lo < p.s_o) {
+          lo = p.s_o;
+          Tlo = T4;
+        }
+      }
+    }
+    double w0 = -1.0;
+    if (m < n_guess && guess[m] > 0.0) w0 = w_of_r(p, guess[m], outbound);
+    double w;
+    const int it = solve_crossing(p, G, lo, Tlo, hi, Thi, w0, tol, &w);
+    if (it < 0) return out;
+    out.iters += it;
+    const double r = 1.0 / path_x(p, w);
+    if (m < 3) out.r[m] = r;
+    out.n = m + 1;
+    w_prev = w;
+    T_prev = G;
+    if (r >= r_in && r <= r_out) {
+      out.hit = m;
+      out.r_hit = r;
+      out.cls = kDisk;
+      out.ok = 1;
+      return out;
+    }
+  }
+  return out;  // too many crossings: not resolved
+}
+
+}  // namespace mino

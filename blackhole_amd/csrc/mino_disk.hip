@@ -1,0 +1,191 @@
+// mino_disk.hip - thin-disk records from the Mino-time crossing solver
+// (mino.hpp) on gfx950: the refinement pass of the surrogate render (the
+// network's crossing radii as starting points, every crossing and the hit
+// class solved exactly) and a network-free tracer of the same records.
+//
+// One ray per lane, 256-lane workgroups over the packed pixel index.  A ray
+// the solver cannot resolve (quadrature or iteration limit, typically within
+// a few 1e-3 M of the critical curve) is flagged BH_FLAG_FALLBACK and
+// appended to the fallback list (one atomic per wave) for the integrator.
+// The record of a solved ray is the integrator's thin-disk record: the first
+// crossing radii, g from the exact Keplerian formula with the ray's xi (0
+// inside r_ISCO), the intensity of the crossing inside [r_ISCO, r_out] from
+// the camera's LUT by the integrator's formula, the hit class, the number of
+// crossings, steps = Newton iterations, flags BH_FLAG_REFINED.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/bhk_trace.h"
+#include "mino.hpp"
+#include "trace_common.hpp"
+
+namespace bhk {
+int fail(const std::string& msg);
+
+__global__ void __launch_bounds__(256) k_mino_disk(const MinoParams P) {
+  const long long gid = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  bool fb = false;
+  if (gid < P.n_total) {
+    const int cam = find_camera(P.cams, P.n_cams, gid);
+    const CamDev& k = P.cams[cam];
+    const long long local = gid - k.offset;
+    BH_PixelRecord* rec = P.out + gid;
+    int* i4 = reinterpret_cast<int*>(rec);
+    double2* d2 = reinterpret_cast<double2*>(rec);
+    int4 tail = make_int4(0, __float_as_int(1.0f), 0, 0);  // class/count/flags, confidence, fp64 steps, samples
+    if (P.refine) tail = *reinterpret_cast<const int4*>(i4 + 28);
+    const unsigned flags_in = (static_cast<unsigned>(tail.x) >> 16) & 0xffu;
+    if (local >= 0 && local < k.w * k.h && !(flags_in & BH_FLAG_FALLBACK)) {
+      const long long py = local / k.w, px = local - py * k.w;
+      const double alpha = k.alpha_min + (static_cast<double>(px) + 0.5) * k.dalpha;
+      const double beta = k.beta_max - (static_cast<double>(py) + 0.5) * k.dbeta;
+      const double a = k.a;
+      const double xi = -(alpha * k.sin_o);
+      const double eta = __builtin_fma(beta, beta, (alpha * alpha - a * a) * (k.cos_o * k.cos_o));
+      double guess[3] = {0.0, 0.0, 0.0};
+      int n_guess = 0;
+      if (P.refine) {
+        const double4 cr = *reinterpret_cast<const double4*>(reinterpret_cast<const double*>(rec) + 4);
+        guess[0] = cr.x;
+        guess[1] = cr.y;
+        guess[2] = cr.z;
+        n_guess = min((tail.x >> 8) & 0xff, 3);
+      }
+      const mino::RaySolution s = mino::solve_ray(a, xi, eta, k.r_obs, k.cos_o, k.sin_o * beta, k.r_stop,
+                                                  P.r_escape, k.r_isco, P.disk_r_out, guess, n_guess, P.tol);
+      if (s.ok) {
+        double g[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[c] = (c < s.n && s.r[c] >= k.r_isco) ? disk_g(s.r[c], a, xi) : 0.0;
+        double I = 0.0, r_exit;
+        if (s.hit >= 0) {
+          const double rh = s.r_hit;
+          const double* lut = P.luts + static_cast<long long>(k.lut_index) * P.lut_n;
+          const double F = lut_linear(lut, P.lut_n, k.r_isco, P.disk_r_out, rh);
+          I = disk_intensity(F, s.hit < 3 ? g[s.hit] : disk_g(rh, a, xi), P.nu, P.T_max);
+          r_exit = rh;
+        } else {
+          r_exit = (s.cls == mino::kHorizon) ? k.r_stop : P.r_escape;
+        }
+        d2[0] = make_double2(I, r_exit);
+        if (!P.refine) d2[1] = make_double2(0.0, 0.0);  // the escape direction stays the network's
+        d2[2] = make_double2(s.r[0], s.r[1]);
+        d2[3] = make_double2(s.r[2], g[0]);
+        d2[4] = make_double2(g[1], g[2]);
+        d2[5] = make_double2(0.0, 0.0);
+        *reinterpret_cast<int4*>(i4 + 24) = make_int4(0, 0, s.iters, 0);  // drift_carter, steps, attempts
+        const unsigned flags = (flags_in & BH_FLAG_SURROGATE) | BH_FLAG_REFINED;
+        tail.x = static_cast<int>(static_cast<unsigned>(s.cls) | (static_cast<unsigned>(min(s.n, 255)) << 8) |
+                                  (flags << 16));
+        *reinterpret_cast<int4*>(i4 + 28) = tail;
+      } else {
+        fb = true;
+        if (!P.refine) {
+          d2[0] = make_double2(0.0, 0.0);
+          d2[1] = make_double2(0.0, 0.0);
+          d2[2] = make_double2(0.0, 0.0);
+          d2[3] = make_double2(0.0, 0.0);
+          d2[4] = make_double2(0.0, 0.0);
+          d2[5] = make_double2(0.0, 0.0);
+          *reinterpret_cast<int4*>(i4 + 24) = make_int4(0, 0, 0, 0);
+        }
+        tail.x = (tail.x & 0xffff) | static_cast<int>((flags_in | BH_FLAG_FALLBACK) << 16);
+        *reinterpret_cast<int4*>(i4 + 28) = tail;
+      }
+    }
+  }
+  wave_append(fb, gid, P.fb_ids, P.fb_count, P.fb_cap);
+}
+
+__global__ void k_mino_zero(unsigned long long* count) {
+  if (threadIdx.x == 0) *count = 0ull;
+}
+
+hipError_t launch_mino_disk(const MinoParams& P, hipStream_t s) {
+  const long long nblk = (P.n_total + 255) / 256;
+  if (nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mino_disk, dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, P);
+  return hipGetLastError();
+}
+
+namespace {
+// Per (device, stream) device copy of the cameras.
+struct CamBuf {
+  CamDev* d = nullptr;
+  size_t cap = 0;
+};
+std::mutex g_mu;
+std::map<std::pair<int, void*>, CamBuf> g_bufs;
+}  // namespace
+
+}  // namespace bhk
+
+using namespace bhk;
+
+extern "C" {
+
+int bh_trace_disk_mino(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts, const double* disk_luts,
+                       int disk_lut_n, BH_PixelRecord* out, int64_t* fallback_ids, uint64_t* fallback_count,
+                       int64_t fallback_cap, void* stream) {
+  if (!cams || n_cams <= 0 || !opts || !disk_luts || disk_lut_n < 2 || !out || !fallback_ids || !fallback_count)
+    return fail("bh_trace_disk_mino: NULL argument");
+  if (fallback_cap <= 0) return fail("bh_trace_disk_mino: fallback_cap must be > 0");
+  if (!opts->disk_enabled || opts->volume_enabled)
+    return fail("bh_trace_disk_mino: thin disk only (disk_enabled = 1, volume_enabled = 0)");
+  if (!(opts->disk_r_out > 0.0) || !(opts->r_escape > 0.0))
+    return fail("bh_trace_disk_mino: disk_r_out and r_escape must be > 0");
+  std::vector<CamDev> hc;
+  long long total = 0;
+  if (make_cam_devs(cams, n_cams, opts->horizon_factor, "bh_trace_disk_mino", &hc, &total) != 0) return -1;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail("bh_trace_disk_mino: no HIP device");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const size_t bytes = hc.size() * sizeof(CamDev);
+  CamDev* d_cams;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    CamBuf& b = g_bufs[{dev, stream}];
+    if (b.cap < bytes) {
+      if (b.d) (void)hipFree(b.d);
+      b.d = nullptr;
+      b.cap = 0;
+      const size_t cap = std::max<size_t>(bytes, 64 * sizeof(CamDev));
+      if (hipMalloc(&b.d, cap) != hipSuccess) return fail("bh_trace_disk_mino: hipMalloc failed");
+      b.cap = cap;
+    }
+    d_cams = b.d;
+  }
+  // Pageable source: the runtime stages it before returning.
+  if (hipMemcpyAsync(d_cams, hc.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+    return fail("bh_trace_disk_mino: camera upload failed");
+  MinoParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.cams = d_cams;
+  P.luts = disk_luts;
+  P.out = out;
+  P.fb_ids = reinterpret_cast<long long*>(fallback_ids);
+  P.fb_count = reinterpret_cast<unsigned long long*>(fallback_count);
+  P.fb_cap = fallback_cap;
+  P.n_total = total;
+  P.disk_r_out = opts->disk_r_out;
+  P.nu = opts->nu_obs;
+  P.T_max = opts->disk_T_max;
+  P.r_escape = opts->r_escape;
+  P.tol = kMinoTol;
+  P.n_cams = n_cams;
+  P.lut_n = disk_lut_n;
+  P.refine = 0;
+  hipLaunchKernelGGL(k_mino_zero, dim3(1), dim3(64), 0, st, P.fb_count);
+  if (total / 256 + 1 > 0x7fffffffll) return fail("bh_trace_disk_mino: too many pixels");
+  const hipError_t e = launch_mino_disk(P, st);
+  return e == hipSuccess ? 0 : fail(std::string("bh_trace_disk_mino: launch failed: ") + hipGetErrorString(e));
+}
+
+}  // extern "C"

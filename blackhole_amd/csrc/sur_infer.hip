@@ -93,6 +93,7 @@ struct InferParams {
   double disk_r_out, nu, T_max, r_escape;
   int n_cams, lut_n, n_hidden, audit_every;  // weight layers 1..n_hidden are 256 x 256
   float ring_band, edge_tol, conf_max, amb;
+  int refine;  // the Mino-time solver follows: only the ring band falls back here
 };
 
 __device__ __forceinline__ uint32_t hash32(uint64_t x) {
@@ -177,26 +178,6 @@ __device__ __forceinline__ void hidden_mfma(WaveAcc& acc, const uint16_t* __rest
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bv, acc[i][j], 0, 0, 0);
     }
-  }
-}
-
-// Appends id to list (flag set) with one atomic per wave; must be called by
-// every lane of the wave.
-__device__ __forceinline__ void wave_append(bool flag, long long id, long long* list,
-                                            unsigned long long* count, long long cap) {
-  const int lane = static_cast<int>(__lane_id());
-  const unsigned long long m = __ballot(flag);
-  if (!m) return;
-  const int leader = __ffsll(static_cast<long long>(m)) - 1;
-  unsigned long long base = 0;
-  if (lane == leader) base = atomicAdd(count, static_cast<unsigned long long>(__popcll(m)));
-  const unsigned int lo = __shfl(static_cast<unsigned int>(base & 0xffffffffull), leader);
-  const unsigned int hi = __shfl(static_cast<unsigned int>(base >> 32), leader);
-  base = (static_cast<unsigned long long>(hi) << 32) | lo;
-  const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  if (flag) {
-    const long long slot = static_cast<long long>(base + __popcll(m & below));
-    if (slot < cap) list[slot] = id;
   }
 }
 
@@ -323,7 +304,7 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
     const float cap = O[BH_SUR_CAPTURE];
     const float conf = O[BH_SUR_CONF];
     const int nc = static_cast<int>(fminf(fmaxf(rintf(ncf), 0.0f), 3.0f));
-    fb = !(conf <= P.conf_max) || fabsf(ncf - rintf(ncf)) > P.amb;
+    fb = !P.refine && (!(conf <= P.conf_max) || fabsf(ncf - rintf(ncf)) > P.amb);
     // critical-curve band: radial distance from the curve's [lo, hi] range in
     // the pixel's polar bin, against the band scaled by the bin's obliquity
     {
@@ -347,7 +328,8 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
         if (c == 0) { cr0 = r; cg0 = g; }
         if (c == 1) { cr1 = r; cg1 = g; }
         if (c == 2) { cr2 = r; cg2 = g; }
-        if (fabs(r - k.r_isco) < P.edge_tol * k.r_isco || fabs(r - P.disk_r_out) < P.edge_tol * P.disk_r_out)
+        if (!P.refine &&
+            (fabs(r - k.r_isco) < P.edge_tol * k.r_isco || fabs(r - P.disk_r_out) < P.edge_tol * P.disk_r_out))
           fb = true;
         if (r >= k.r_isco && r <= P.disk_r_out) {
           hit = c;
@@ -357,7 +339,7 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
         }
       }
     }
-    if (hit < 0 && fabsf(cap - rintf(cap)) > P.amb) fb = true;
+    if (!P.refine && hit < 0 && fabsf(cap - rintf(cap)) > P.amb) fb = true;
     const int cls = hit >= 0 ? BH_HIT_DISK : (cap > 0.5f ? BH_HIT_HORIZON : BH_HIT_ESCAPED);
     const int ncr = hit >= 0 ? hit + 1 : nc;
     double r_exit = 0.0, u_exit = 0.0, phi_exit = 0.0;
@@ -661,11 +643,32 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
   P.edge_tol = rp->edge_tol;
   P.conf_max = rp->conf_max;
   P.amb = rp->ambiguity;
+  P.refine = rp->refine != 0;
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P.fb_count, P.audit_count);
   const long long nblk = (total + kTile - 1) / kTile;
   if (nblk > 0x7fffffffll) return fail("bh_surrogate_render: too many pixels");
   hipLaunchKernelGGL(k_sur_render, dim3(static_cast<unsigned>(nblk)), dim3(256), 0, st, P);
-  const hipError_t e = hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && P.refine) {
+    MinoParams M;
+    std::memset(&M, 0, sizeof(M));
+    M.cams = d_cams;
+    M.luts = disk_luts;
+    M.out = out;
+    M.fb_ids = P.fb_ids;
+    M.fb_count = P.fb_count;
+    M.fb_cap = fallback_cap;
+    M.n_total = total;
+    M.disk_r_out = opts->disk_r_out;
+    M.nu = opts->nu_obs;
+    M.T_max = opts->disk_T_max;
+    M.r_escape = opts->r_escape;
+    M.tol = kMinoTol;
+    M.n_cams = n_cams;
+    M.lut_n = disk_lut_n;
+    M.refine = 1;
+    e = launch_mino_disk(M, st);
+  }
   return e == hipSuccess ? 0 : fail(std::string("bh_surrogate_render: launch failed: ") + hipGetErrorString(e));
 }
 

@@ -159,6 +159,43 @@ __device__ __forceinline__ long long refill_id(bool need, unsigned long long* he
   return id;
 }
 
+// Appends id to list (flag set) with one atomic per wave; must be called by
+// every lane of the wave.
+__device__ __forceinline__ void wave_append(bool flag, long long id, long long* list,
+                                            unsigned long long* count, long long cap) {
+  const int lane = static_cast<int>(__lane_id());
+  const unsigned long long m = __ballot(flag);
+  if (!m) return;
+  const int leader = __ffsll(static_cast<long long>(m)) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(count, static_cast<unsigned long long>(__popcll(m)));
+  const unsigned int lo = __shfl(static_cast<unsigned int>(base & 0xffffffffull), leader);
+  const unsigned int hi = __shfl(static_cast<unsigned int>(base >> 32), leader);
+  base = (static_cast<unsigned long long>(hi) << 32) | lo;
+  const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  if (flag) {
+    const long long slot = static_cast<long long>(base + __popcll(m & below));
+    if (slot < cap) list[slot] = id;
+  }
+}
+
+// Thin-disk records from the Mino-time crossing solver (mino_disk.hip):
+// every pixel (refine = 0) or, over surrogate records, the rays not flagged
+// for fallback with the record's crossing radii as starting points
+// (refine = 1).  Unresolved rays are appended to fb_ids.
+struct MinoParams {
+  const CamDev* cams;
+  const double* luts;
+  BH_PixelRecord* out;
+  long long* fb_ids;
+  unsigned long long* fb_count;
+  long long fb_cap, n_total;
+  double disk_r_out, nu, T_max, r_escape, tol;
+  int n_cams, lut_n, refine, pad;
+};
+constexpr double kMinoTol = 1e-10;  // relative tolerance of the radial quadrature
+hipError_t launch_mino_disk(const MinoParams& P, hipStream_t s);
+
 // Host: device constants of n_cams cameras with their validation (trace.hip);
 // -1 with the error set, prefixed by who, on an invalid camera.
 int make_cam_devs(const BH_Camera* cams, int n_cams, double horizon_factor, const char* who,

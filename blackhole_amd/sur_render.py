@@ -2,7 +2,11 @@
 
 One frame set (any number of cameras at r_obs = 100 M) goes through:
   1. bh_surrogate_render: every pixel through the fused BF16-MFMA network
-     and decode (records, fallback id list, audit id list);
+     and decode (records, fallback id list, audit id list); with refine set
+     (the default) each ray outside the photon-ring band then has its
+     crossings and hit class solved exactly from the Mino-time integrals of
+     motion, starting from the network's radii, and the rays that solver
+     cannot resolve join the fallback list;
   2. bh_trace_pixel_list: the fallback rays traced by the plain integrator
      into the same image (records overwrite the surrogate's);
   3. bh_trace_pixel_list_compact + bh_surrogate_audit: the audit sample
@@ -30,6 +34,7 @@ class RenderConfig:
     conf_max: float = 0.0       # log10 of the predicted squared error (0: effectively off)
     ambiguity: float = 0.3
     audit_every: int = 100      # 1 % audit
+    refine: bool = True         # exact Mino-time crossing solve after the network
     intensity_floor_rel: float = 1e-6  # audit compares pixels with I_ref > floor x max I
 
 
@@ -64,7 +69,7 @@ class SurrogateRenderer:
         self.stats = torch.zeros(8, dtype=torch.float64, device=dev)
         self.ring = torch.from_numpy(ring_tables(scene.cams).reshape(-1)).to(dev)
         self.params = S.RenderParams(cfg.ring_band, cfg.edge_tol, cfg.conf_max, cfg.ambiguity,
-                                     cfg.audit_every, 0)
+                                     cfg.audit_every, int(cfg.refine))
         self.floor_I = 0.0
 
     def _stream(self):
@@ -114,3 +119,32 @@ class SurrogateRenderer:
         d["mean_rel"] = d["sum_rel"] / max(d["n_cmp"], 1.0)
         d["flux_weighted"] = d["sum_abs_dI"] / max(d["sum_I_ref"], 1e-300)
         return d
+
+
+class MinoRenderer:
+    """Network-free thin-disk images: the Mino-time crossing solver over every
+    pixel (bh_trace_disk_mino), then the plain integrator over the rays the
+    solver leaves (bh_trace_pixel_list into the same records)."""
+
+    def __init__(self, scene: tracer.DeviceScene):
+        import torch
+        self.torch = torch
+        self.sc = scene
+        n = scene.n_records
+        dev = scene.luts.device
+        self.out = tracer.alloc_records(n, device=dev)
+        self.fb_ids = torch.empty(max(n, 1024), dtype=torch.int64, device=dev)
+        self.count = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def solve(self) -> None:
+        tracer.trace_disk_mino(self.sc, self.out, self.fb_ids, self.count)
+
+    def fallback(self) -> None:
+        tracer.launch_ids(self.sc, self.fb_ids, self.count, self.out)
+
+    def run(self) -> None:
+        self.solve()
+        self.fallback()
+
+    def fallback_count(self) -> int:
+        return int(self.count.cpu()[0])

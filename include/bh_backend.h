@@ -36,8 +36,9 @@ int bhCudaResize(BH_CudaBackend* backend, unsigned int glTexture, int width, int
 int bhCudaRegisterLut(BH_CudaBackend* backend, int slot, unsigned int glTexture, unsigned int target);
 int bhCudaRegisterBackgroundTexture(BH_CudaBackend* backend, unsigned int glTexture, unsigned int target);
 
-/* cuda_backend.h:95-103: variant < 0 or out of range selects
- * bh_select_kernel_variant(); returns the active variant (-1 for NULL). */
+/* cuda_backend.h:95-103: variant < 0 selects bh_select_kernel_variant(),
+ * variant >= BH_KERNEL_COUNT falls back to BH_KERNEL_FP32_BASELINE (as the
+ * reference, cuda_backend.cu:144-154); returns the active variant (-1 for NULL). */
 int bhCudaSetVariant(BH_CudaBackend* backend, int variant);
 int bhCudaGetVariant(BH_CudaBackend* backend);
 

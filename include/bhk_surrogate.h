@@ -112,7 +112,11 @@ typedef struct BH_SurrogateRenderParams {
     float ambiguity;      /* fall back when count / capture outputs are this far from
                              an integer                                              */
     int32_t audit_every;  /* 1-in-N deterministic audit sample (0: none)            */
-    int32_t pad0;
+    int32_t refine;       /* 1: solve every non-fallback ray's crossings exactly from
+                             the Mino-time integrals (bh_trace_disk_mino's solver) with
+                             the network's radii as starting points; confidence,
+                             count and edge criteria are then not applied, only the
+                             ring band and the solver's own failures fall back      */
 } BH_SurrogateRenderParams;
 
 /* Render every pixel of the cameras with the fused surrogate kernel

@@ -71,6 +71,7 @@ enum BH_Precision {
 #define BH_FLAG_FALLBACK      0x02u /* surrogate deferred this ray to the tracer */
 #define BH_FLAG_AUDITED       0x04u /* ray traced for the surrogate audit        */
 #define BH_FLAG_SURROGATE     0x08u /* record produced by the surrogate          */
+#define BH_FLAG_REFINED       0x10u /* crossings solved by the Mino-time solver   */
 
 /* ---------------------------------------------------------------------- */
 /* Inputs                                                                  */
@@ -335,6 +336,24 @@ int bh_trace_rays_host(const BH_RaySpec* rays, int64_t n, const BH_TraceOptions*
  * HIP events recorded around the launch on its stream (ms; -1 if none).  The
  * caller must have synchronised the stream. */
 float bh_last_kernel_ms(void);
+
+/* Thin-disk records without step integration: the equatorial crossings of
+ * every pixel's geodesic from the Mino-time integrals of motion (polar
+ * elliptic integrals in closed form, the radial integral by adaptive
+ * Gauss-Kronrod quadrature through the turning point, each crossing by a
+ * bracketed Newton iteration), then the integrator's disk record (hit class,
+ * first three crossing radii, Keplerian g, LUT intensity; steps = Newton
+ * iterations, flags BH_FLAG_REFINED).  Same cameras, options and LUTs as
+ * bh_trace_image_plane with disk_enabled = 1 and volume_enabled = 0; the
+ * escape direction (u_exit, phi_exit) is not computed (0).  Rays the solver
+ * does not resolve (within about 1e-3 M of the critical curve) get
+ * BH_FLAG_FALLBACK and their ids in fallback_ids (count in device memory,
+ * zeroed by the call) for bh_trace_pixel_list.  No counterpart in the
+ * reference; an exact, integration-free method for the thin disk. */
+int bh_trace_disk_mino(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+                       const double* disk_luts, int disk_lut_n, BH_PixelRecord* out,
+                       int64_t* fallback_ids, uint64_t* fallback_count, int64_t fallback_cap,
+                       void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,91 @@
+"""The Mino-time thin-disk solver on the GPU (bh_trace_disk_mino and the
+refinement pass of the surrogate render) against the oracle's step-integrated
+records: identical hit classes, crossing radii within 1e-6, intensities
+within 1e-6 relative on average; unresolved rays handed to the integrator carry its
+records bit for bit."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+from blackhole_amd import abi, sur_render, tracer, workloads
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "blackhole_amd", "weights", "surrogate_c5.npz")
+
+
+def _check_disk_records(got, ref, solved):
+    assert np.array_equal(got["hit_class"][solved], ref["hit_class"][solved])
+    disk = solved & (ref["hit_class"] == abi.HIT_DISK)
+    I_ref = ref["intensity"]
+    m = disk & (I_ref > 1e-6 * I_ref.max())
+    rel_I = np.abs(got["intensity"][m] - I_ref[m]) / I_ref[m]
+    # the flux vanishes at r_ISCO, so next to the disk's inner edge a radius
+    # difference of the integrator's interpolation size (1e-7) is a relative
+    # intensity difference up to a few 1e-5; elsewhere it is below 1e-7
+    assert rel_I.mean() <= 1e-6 and rel_I.max() <= 1e-3, (rel_I.mean(), rel_I.max())
+    nc = np.minimum(np.minimum(got["n_cross"], ref["n_cross"]), 3)
+    worst = 0.0
+    for k in range(3):
+        sel = solved & (nc > k)
+        worst = max(worst, float((np.abs(got["cross_r"][sel, k] - ref["cross_r"][sel, k])
+                                  / ref["cross_r"][sel, k]).max(initial=0.0)))
+    assert worst <= 1e-6, worst
+    return float(rel_I.mean())
+
+
+def test_mino_disk_matches_oracle_and_falls_back_exactly():
+    import torch
+    w = workloads.c5(40)  # the 16 frames of the surrogate comparison at 40 x 40
+    print("[mino] 16 x 40^2 rays", flush=True)
+    sc = tracer.upload_scene(w["cams"], w["opts"], w["luts"])
+    R = sur_render.MinoRenderer(sc)
+    R.solve()
+    torch.cuda.synchronize()
+    got = tracer.records_to_numpy(R.out)
+    ref = OL.trace(w["cams"], w["opts"], luts=w["luts"], lut_n=w["luts"].shape[1])
+    fb = R.fallback_count()
+    flagged = (got["flags"] & abi.FLAG_FALLBACK) != 0
+    assert fb == int(flagged.sum()) and fb <= sc.n_records // 200
+    assert ((got["flags"][~flagged] & abi.FLAG_REFINED) != 0).all()
+    _check_disk_records(got, ref, ~flagged)
+    R.fallback()
+    torch.cuda.synchronize()
+    after = tracer.records_to_numpy(R.out)
+    ids = R.fb_ids[:fb].cpu().numpy()
+    for f in ("intensity", "r_exit", "u_exit", "phi_exit", "cross_r", "hit_class", "steps"):
+        assert np.array_equal(after[f][ids], ref[f][ids]), f
+    assert np.array_equal(after["hit_class"], ref["hit_class"])
+
+
+def test_refined_surrogate_render_matches_oracle():
+    """Surrogate + refinement + fallback on four of the frames: every record
+    at the oracle's class and its intensity; with the
+    network's radii as starting points the solver needs fewer Newton
+    iterations than from the bracket alone."""
+    import torch
+    from blackhole_amd import surrogate_train as ST
+    w = workloads.c5(48, spins=(0.0, 0.998), inclinations=(17.0, 85.0))
+    print("[refine] 4 x 48^2 rays", flush=True)
+    sc = tracer.upload_scene(w["cams"], w["opts"], w["luts"])
+    sur = ST.load(WEIGHTS)
+    R = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig())
+    R.run(audit=False)
+    M = sur_render.MinoRenderer(sc)
+    M.solve()
+    torch.cuda.synchronize()
+    got = tracer.records_to_numpy(R.out)
+    ref = OL.trace(w["cams"], w["opts"], luts=w["luts"], lut_n=w["luts"].shape[1])
+    assert np.array_equal(got["hit_class"], ref["hit_class"])
+    _check_disk_records(got, ref, np.ones(len(got), bool))
+    refined = (got["flags"] & abi.FLAG_REFINED) != 0
+    assert refined.mean() > 0.9
+    fresh = tracer.records_to_numpy(M.out)
+    both = refined & ((fresh["flags"] & abi.FLAG_REFINED) != 0)
+    it_guess, it_fresh = got["steps"][both].mean(), fresh["steps"][both].mean()
+    print("[refine] Newton iterations per ray: network start", it_guess, "bracket start", it_fresh, flush=True)
+    assert it_guess < it_fresh

@@ -52,7 +52,8 @@ def test_backend_abi_renders_into_its_framebuffer():
     try:
         assert L.bhCudaGetVariant(b) == 2
         assert L.bhCudaSetVariant(b, 0) == 0 and L.bhCudaGetVariant(b) == 0
-        assert L.bhCudaSetVariant(b, 99) == 2  # out of range -> the selected default
+        assert L.bhCudaSetVariant(b, 99) == 0  # beyond the table -> FP32 baseline
+        assert L.bhCudaSetVariant(b, -1) == 2  # negative -> the selected default
         assert L.bhCudaRegisterLut(b, 3, 11, 0) == 0 and L.bhCudaRegisterBackgroundTexture(b, 12, 0) == 0
         assert L.bhCudaGlTexture(b, -1) == 7 and L.bhCudaGlTexture(b, 3) == 11 and L.bhCudaGlTexture(b, 8) == 12
         p = tracer.launch_params(width=32, height=24)

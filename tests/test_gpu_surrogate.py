@@ -105,7 +105,7 @@ def test_fused_render_matches_layerwise_forward_checker_and_decode():
                                   nu_obs=2.417989242e17, disk_T_max=1.0e7)
     luts = tracer.disk_luts(cams, 20.0)
     sc = tracer.upload_scene(cams, opts, luts)
-    R = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(ring_band=0.25, audit_every=7))
+    R = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(ring_band=0.25, audit_every=7, refine=False))
     R.out.fill_(0xAB)
     R.render()
     torch.cuda.synchronize()
