@@ -45,6 +45,7 @@ using std::acos;
 using std::cbrt;
 using std::cos;
 using std::fabs;
+using std::fma;
 using std::fmax;
 using std::fmin;
 using std::sqrt;
@@ -167,6 +168,19 @@ struct Path {
   int turning;
 };
 
+// 1 / sqrt(v), v > 0: on the device the hardware reciprocal square root
+// (v_rsq_f64) and one Newton step y (3/2 - v y^2 / 2), relative error about
+// 1e-13 -- far below the quadrature tolerance -- for a fraction of the
+// correctly rounded sqrt-and-divide sequence; on the host 1 / sqrt(v).
+MINO_HD double inv_sqrt(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(v);
+  return y * (1.5 - 0.5 * v * (y * y));
+#else
+  return 1.0 / sqrt(v);
+#endif
+}
+
 MINO_HD double path_x(const Path& p, double w) {
   if (p.turning) {
     const double s = p.s_o - w;
@@ -178,13 +192,13 @@ MINO_HD double path_x(const Path& p, double w) {
 MINO_HD double path_f(const Path& p, double w) {
   if (p.turning) {
     const double s = p.s_o - w;
-    const double x = p.x4 - s * s;
-    const double S = -(((p.q3 * x + p.q2) * x + p.q1) * x + p.q0);
-    return S > 0.0 ? 2.0 / sqrt(S) : 1e300;
+    const double x = fma(-s, s, p.x4);
+    const double S = -fma(fma(fma(p.q3, x, p.q2), x, p.q1), x, p.q0);
+    return S > 0.0 ? 2.0 * inv_sqrt(S) : 1e300;
   }
   const double x = p.x_o + w;
-  const double Rx = 1.0 + x * x * (p.A + x * (p.B + p.C * x));
-  return Rx > 0.0 ? 1.0 / sqrt(Rx) : 1e300;
+  const double Rx = fma(x * x, fma(x, fma(p.C, x, p.B), p.A), 1.0);
+  return Rx > 0.0 ? inv_sqrt(Rx) : 1e300;
 }
 
 // One Gauss-Kronrod 7-15 panel (the QUADPACK qk15 rule: Kronrod abscissae
@@ -208,8 +222,8 @@ MINO_HD double gk15(const Path& p, double a, double b, double* err) {
   for (int j = 0; j < 7; ++j) {
     const double dx = h * xgk[j];
     const double f = path_f(p, c - dx) + path_f(p, c + dx);
-    rk += wgk[j] * f;
-    if (j & 1) rg += wg[j >> 1] * f;
+    rk = fma(wgk[j], f, rk);
+    if (j & 1) rg = fma(wg[j >> 1], f, rg);
   }
   *err = fabs((rk - rg) * h);
   return rk * h;
@@ -287,7 +301,24 @@ MINO_HD int solve_crossing(const Path& p, double G, double w_lo, double T_lo, do
                            double tol, double* w_out) {
   double lo = w_lo, Tlo = T_lo, hi = w_hi, Thi = T_hi;
   double w = w0;
-  if (!(w > lo && w < hi)) w = lo + (hi - lo) * ((G - Tlo) / (Thi - Tlo));
+  if (!(w > lo && w < hi)) {
+    // no usable start: invert the cubic Hermite model of T on the bracket
+    // (values and slopes f at both ends), from the linear interpolant
+    const double L = hi - lo;
+    const double d0 = path_f(p, lo) * L, d1 = path_f(p, hi) * L, dT = Thi - Tlo;
+    const double target = (G - Tlo) / dT;
+    double t = target;
+    for (int k = 0; k < 4; ++k) {
+      const double t2 = t * t, t3 = t2 * t;
+      const double h = (3.0 * t2 - 2.0 * t3) + ((t3 - 2.0 * t2 + t) * d0 + (t3 - t2) * d1) / dT;
+      const double dh = (6.0 * t - 6.0 * t2) + ((3.0 * t2 - 4.0 * t + 1.0) * d0 + (3.0 * t2 - 2.0 * t) * d1) / dT;
+      if (!(dh > 0.0)) break;
+      t -= (h - target) / dh;
+      t = fmin(fmax(t, 0.0), 1.0);
+    }
+    w = lo + L * t;
+    if (!(w > lo && w < hi)) w = lo + L * target;
+  }
   int ok = 1;
   // T at the starting point from the nearer end of the bracket
   double T;
@@ -312,6 +343,12 @@ MINO_HD int solve_crossing(const Path& p, double G, double w_lo, double T_lo, do
     }
     const double f = path_f(p, w);
     double wn = w + dT / f;
+    // close enough that the Newton step's own error (quadratic in dT) is
+    // negligible: take it without another quadrature
+    if (fabs(dT) <= 1e-6 * G && wn > lo && wn < hi) {
+      *w_out = wn;
+      return it;
+    }
     if (!(wn > lo && wn < hi)) wn = 0.5 * (lo + hi);
     const double inc = (wn > w) ? integrate(p, w, wn, tol, &ok) : -integrate(p, wn, w, tol, &ok);
     if (!ok) return -1;

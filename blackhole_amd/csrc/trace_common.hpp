@@ -193,7 +193,7 @@ struct MinoParams {
   double disk_r_out, nu, T_max, r_escape, tol;
   int n_cams, lut_n, refine, pad;
 };
-constexpr double kMinoTol = 1e-10;  // relative tolerance of the radial quadrature
+constexpr double kMinoTol = 1e-8;  // relative tolerance of the G7 error estimate in the radial quadrature
 hipError_t launch_mino_disk(const MinoParams& P, hipStream_t s);
 
 // Host: device constants of n_cams cameras with their validation (trace.hip);
