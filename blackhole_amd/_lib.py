@@ -2,7 +2,7 @@
 
 There is no CPU fallback: if the library is missing or cannot be loaded, every
 entry point raises.  Building happens in ``blackhole_amd.build`` (called by
-``__graft_entry__.build()``), never implicitly on a GPU box.
+``__graft_entry__.build()``), never implicitly at import time.
 """
 from __future__ import annotations
 
