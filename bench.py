@@ -48,6 +48,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "rays/s at ≤1e-3 rel intensity err: surrogate+fallback vs plain GPU DP45"
+# HBM bytes per ray of the c5 kernels, measured with rocprofv3 --pmc FETCH_SIZE and
+# --pmc WRITE_SIZE on this workload (bench.py --copy-calibration: FETCH_SIZE reads a 512 MiB
+# copy as 256 MiB, so fetches are scaled x 2; WRITE_SIZE reads it exactly).
+TRAFFIC_PER_RAY = {
+    "k_sur_render": (2 * 3530.0 + 8425000.0) * 1024 / 67108864,            # 128.6 B: the record + lists
+    "k_trace_fp64_dp45_disk": (2 * 314.0 + 12408500.0) * 1024 / 67108864,  # 189.3 B: record + crossing slots
+}
+TRAFFIC_NOTE = "rocprofv3 FETCH_SIZE (x2, copy-calibrated) + WRITE_SIZE per dispatch, per ray x rays"
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 BF16_DENSE_PEAK_TFLOPS = 2500.0
 WEIGHTS = os.path.join(ROOT, "blackhole_amd", "weights", "surrogate_c5.npz")
@@ -239,13 +247,17 @@ def bench_c5(args):
     f_sur = f_ray * n
     roof_plain = {
         "bound": "valu", "achieved": f_plain / (plain_kernel_ms / 1000.0) / 1e12,
-        "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s", "traffic": None,
+        "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "traffic": TRAFFIC_PER_RAY["k_trace_fp64_dp45_disk"] * n if args.size == 2048 else None,
+        "traffic_source": TRAFFIC_NOTE, "algorithmic_bytes": 128.0 * n,
         "kernel": "k_trace_fp64<DP45, no volume>", "kernel_ms": plain_kernel_ms, "flop_per_launch": f_plain,
     }
     roof_plain["frac"] = roof_plain["achieved"] / roof_plain["peak"]
     roof_sur = {
         "bound": "mfma", "achieved": f_sur / (render_ev_ms / 1000.0) / 1e12,
-        "peak": BF16_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s", "traffic": None,
+        "peak": BF16_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "traffic": TRAFFIC_PER_RAY["k_sur_render"] * n if args.size == 2048 else None,
+        "traffic_source": TRAFFIC_NOTE, "algorithmic_bytes": 128.0 * n,
         "kernel": "k_sur_render (BF16 MFMA, FP32 accumulation)", "kernel_ms": render_ev_ms,
         "flop_per_launch": f_sur, "flop_per_ray": f_ray,
     }

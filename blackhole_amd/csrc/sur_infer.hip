@@ -3,20 +3,23 @@
 // k_sur_render: one workgroup (4 waves) takes a tile of 128 consecutive
 // pixels through the whole network without leaving the CU.  The layers are
 // computed transposed, C^T[out][ray] = W[out][k] H[ray][k]^T, so that the
-// weights (row-major [out][in], BF16, streamed from L2) are the MFMA A
-// operand and the activations (BF16 [ray][feature], resident in LDS) the B
-// operand; a lane's four accumulator rows are then four consecutive features
-// of one ray and go back to LDS as one 8-byte store.  Wave w owns output
-// features [64 w, 64 w + 64): 4 feature blocks x 8 ray blocks = 32 tiles of
-// v_mfma_f32_16x16x32_bf16 (128 accumulator registers per lane), so every
-// weight matrix is read once per 128 rays.  LDS per workgroup is 74 KiB, so
-// two workgroups share a CU and one's SiLU epilogue (VALU) overlaps the
-// other's MFMA work.
+// weights (BF16, streamed from L2) are the MFMA A operand and the
+// activations (BF16 [ray][feature], resident in LDS) the B operand.  Wave w
+// owns output features [64 w, 64 w + 64): 2 feature blocks x 4 ray blocks =
+// 8 tiles of v_mfma_f32_32x32x16_bf16 (128 accumulator registers per lane),
+// so every weight matrix is read once per 128 rays.  The 32x32x16 form
+// keeps the matrix pipe near full from a single wave (one 16x16x32 stream
+// per wave leaves half of it idle), which matters because the two waves of
+// a SIMD alternate between MFMA work and their SiLU epilogues.  A lane's
+// accumulator holds, for one ray, four groups of four consecutive features;
+// each group goes back to LDS as one 8-byte store.  The hidden weights are
+// read from a copy swizzled into fragment order (k_swizzle), so each wave's
+// weight fragment is one contiguous 1 KiB load.  LDS per workgroup is 74 KiB,
+// so two workgroups share a CU.
 //
 // Layer 0 (16 features -> 256) keeps the inputs' FP32 resolution: features
-// x and weights W are split into BF16 hi/lo parts and the layer is two MFMA
-// K-steps, [W_hi | W_hi] . [x_hi | x_lo] + [W_lo | 0] . [x_hi | x_lo]
-// = W_hi x_hi + W_hi x_lo + W_lo x_hi (error ~2^-16 relative).
+// x and weights W are split into BF16 hi/lo parts and the layer is three
+// MFMA K-steps, W_hi x_hi + W_hi x_lo + W_lo x_hi (error ~2^-16 relative).
 //
 // The network runs rescaled by L = log2(e): with W_0 and every hidden bias
 // multiplied by L, each accumulator holds z' = L z, SiLU becomes
@@ -68,6 +71,7 @@ namespace sinf {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(16))) float f32x16;
 
 constexpr int kTile = 128;    // rays per workgroup
 constexpr int kW = 256;       // hidden width of the fused kernel
@@ -83,6 +87,7 @@ struct InferParams {
   const uint16_t* w0cat;  // [256][64]: [W_hi | W_hi | W_lo | 0] of L W_0
   const float* bs;        // L b_l for l = 0..n_hidden, [n_hidden + 1][256]
   const uint16_t* w[kMaxLayers];
+  const uint16_t* wsw;    // hidden layers 1..n_hidden in fragment order (k_swizzle), 128 KiB each
   const float* b[kMaxLayers];
   BH_PixelRecord* out;
   long long* fb_ids;
@@ -122,61 +127,79 @@ __device__ __forceinline__ float silu_scaled(float zs) {
   return zs * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-zs));
 }
 
-// A wave's accumulators: acc[i][j] holds (rescaled) features
-// o0 + 16 i + 4 q + e (q = lane / 16, e = 0..3) of rays 16 j + lane % 16.
-typedef f32x4 WaveAcc[4][8];
+// v_mfma_f32_32x32x16_bf16 operand layout (lane l): A[m = l % 32][k = 8 (l / 32) + i],
+// B[k = 8 (l / 32) + i][n = l % 32], accumulator element e at
+// C[m = 8 (e / 4) + 4 (l / 32) + e % 4][n = l % 32].
+// A wave's accumulators: acc[ob][rb] holds (rescaled) features
+// o0 + 32 ob + 8 (e / 4) + 4 (lane / 32) + e % 4 of ray 32 rb + lane % 32.
+typedef f32x16 WaveAcc[2][4];
 
-// The wave's bias slice: bias4[i] = bias[o0 + 16 i + 4 q .. + 3].
-__device__ __forceinline__ void load_bias(f32x4 (&b4)[4], const float* __restrict__ bias, int o0, int lq) {
+// The wave's bias slice in accumulator order.
+__device__ __forceinline__ void load_bias(f32x16 (&b16)[2], const float* __restrict__ bias, int o0, int lh) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const float4 bb = *reinterpret_cast<const float4*>(bias + o0 + 16 * i + 4 * lq);
-    b4[i] = f32x4{bb.x, bb.y, bb.z, bb.w};
-  }
-}
-
-// H[ray][f0 .. f0 + 3] = BF16(L silu) of every tile of the wave.
-__device__ __forceinline__ void store_act(const WaveAcc& acc, uint16_t* __restrict__ H, int o0, int lr, int lq) {
+  for (int ob = 0; ob < 2; ++ob)
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int f0 = o0 + 16 * i + 4 * lq;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      uint2 v;
-      v.x = pack_bf16x2(silu_scaled(acc[i][j][0]), silu_scaled(acc[i][j][1]));
-      v.y = pack_bf16x2(silu_scaled(acc[i][j][2]), silu_scaled(acc[i][j][3]));
-      *reinterpret_cast<uint2*>(H + (16 * j + lr) * kLdH + f0) = v;
+    for (int g = 0; g < 4; ++g) {
+      const float4 bb = *reinterpret_cast<const float4*>(bias + o0 + 32 * ob + 8 * g + 4 * lh);
+      b16[ob][4 * g] = bb.x;
+      b16[ob][4 * g + 1] = bb.y;
+      b16[ob][4 * g + 2] = bb.z;
+      b16[ob][4 * g + 3] = bb.w;
     }
-  }
 }
 
-// One hidden layer's accumulation: acc = bias + W[o0 + ..][:] . H[ray][:]^T.
-// The first K step reads the bias as the MFMA's C operand, so no register
-// is initialised separately.
-__device__ __forceinline__ void hidden_mfma(WaveAcc& acc, const uint16_t* __restrict__ W,
-                                            const uint16_t* __restrict__ H, const f32x4 (&b4)[4], int o0, int lr,
-                                            int lq) {
+// H[ray][f .. f + 3] = BF16(L silu) of every tile of the wave.
+__device__ __forceinline__ void store_act(const WaveAcc& acc, uint16_t* __restrict__ H, int o0, int l32, int lh) {
+#pragma unroll
+  for (int ob = 0; ob < 2; ++ob)
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      uint16_t* row = H + (32 * rb + l32) * kLdH + o0 + 32 * ob + 4 * lh;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        uint2 v;
+        v.x = pack_bf16x2(silu_scaled(acc[ob][rb][4 * g]), silu_scaled(acc[ob][rb][4 * g + 1]));
+        v.y = pack_bf16x2(silu_scaled(acc[ob][rb][4 * g + 2]), silu_scaled(acc[ob][rb][4 * g + 3]));
+        *reinterpret_cast<uint2*>(row + 8 * g) = v;
+      }
+    }
+}
+
+// One hidden layer's accumulation: acc = bias + W[o0 + ..][:] . H[ray][:]^T,
+// 16 K-steps of 16.  Wsw is the layer in fragment order: for feature block
+// fb = o0 / 32 + ob and K-step ks, lane l's 8 weights sit at
+// Wsw[((fb * 16 + ks) * 64 + l) * 8].  The first K-step reads the bias as the
+// MFMA's C operand; the next K-step's weight fragments load one step ahead.
+__device__ __forceinline__ void hidden_mfma(WaveAcc& acc, const uint16_t* __restrict__ Wsw,
+                                            const uint16_t* __restrict__ H, const f32x16 (&b16)[2], int o0,
+                                            int lane, int l32, int lh) {
+  const uint16_t* wp0 = Wsw + ((o0 / 32) * 16 * 64 + lane) * 8;
+  const uint16_t* wp1 = wp0 + 16 * 64 * 8;
+  const uint16_t* hp = H + l32 * kLdH + 8 * lh;
+  bf16x8 an0 = ld16(wp0), an1 = ld16(wp1);
   {
-    bf16x8 a[4];
+    const bf16x8 a0 = an0, a1 = an1;
+    an0 = ld16(wp0 + 512);
+    an1 = ld16(wp1 + 512);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = ld16(W + (o0 + 16 * i + lr) * kW + lq * 8);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const bf16x8 bv = ld16(H + (16 * j + lr) * kLdH + lq * 8);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bv, b4[i], 0, 0, 0);
+    for (int rb = 0; rb < 4; ++rb) {
+      const bf16x8 bv = ld16(hp + 32 * rb * kLdH);
+      acc[0][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bv, b16[0], 0, 0, 0);
+      acc[1][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bv, b16[1], 0, 0, 0);
     }
   }
-#pragma unroll 2  // full unrolling hoists every weight load and spills
-  for (int kc = 32; kc < kW; kc += 32) {
-    bf16x8 a[4];
+#pragma unroll 3
+  for (int ks = 1; ks < kW / 16; ++ks) {
+    const bf16x8 a0 = an0, a1 = an1;
+    if (ks + 1 < kW / 16) {
+      an0 = ld16(wp0 + (ks + 1) * 512);
+      an1 = ld16(wp1 + (ks + 1) * 512);
+    }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = ld16(W + (o0 + 16 * i + lr) * kW + kc + lq * 8);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const bf16x8 bv = ld16(H + (16 * j + lr) * kLdH + kc + lq * 8);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bv, acc[i][j], 0, 0, 0);
+    for (int rb = 0; rb < 4; ++rb) {
+      const bf16x8 bv = ld16(hp + 32 * rb * kLdH + 16 * ks);
+      acc[0][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bv, acc[0][rb], 0, 0, 0);
+      acc[1][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bv, acc[1][rb], 0, 0, 0);
     }
   }
 }
@@ -185,7 +208,8 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
   __shared__ __attribute__((aligned(16))) uint16_t H[kTile * kLdH];  // activations, 66 KiB
   __shared__ __attribute__((aligned(16))) uint16_t X[kTile * 32];    // [x_hi | x_lo]; later FP32 outputs [128][16]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int lr = lane & 15, lq = lane >> 4;
+  const int lr = lane & 15, lq = lane >> 4;    // 16x16x32 layout (output layer)
+  const int l32 = lane & 31, lh = lane >> 5;   // 32x32x16 layout (layer 0, hidden layers)
   const long long base = static_cast<long long>(blockIdx.x) * kTile;
 
   // ---- ray setup and features (threads 0..127) -----------------------------
@@ -230,37 +254,37 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
 
   const int o0 = wave * 64;
   WaveAcc acc;
+  f32x16 b16[2];
 
-  f32x4 b4[4];
-
-  // ---- layer 0: two K-steps over [x_hi | x_lo] ----------------------------------
-  load_bias(b4, P.bs, o0, lq);
+  // ---- layer 0: W_hi x_hi + W_hi x_lo + W_lo x_hi over the [x_hi | x_lo] rows ---
+  load_bias(b16, P.bs, o0, lh);
   {
-    bf16x8 a0[4], a1[4];
+    bf16x8 a[2][3];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      a0[i] = ld16(P.w0cat + (o0 + 16 * i + lr) * 64 + lq * 8);
-      a1[i] = ld16(P.w0cat + (o0 + 16 * i + lr) * 64 + 32 + lq * 8);
-    }
+    for (int ob = 0; ob < 2; ++ob)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const bf16x8 bv = ld16(X + (16 * j + lr) * 32 + lq * 8);
+      for (int c = 0; c < 3; ++c) a[ob][c] = ld16(P.w0cat + (o0 + 32 * ob + l32) * 64 + 16 * c + 8 * lh);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[i], bv, b4[i], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[i], bv, acc[i][j], 0, 0, 0);
+    for (int rb = 0; rb < 4; ++rb) {
+      const bf16x8 xh = ld16(X + (32 * rb + l32) * 32 + 8 * lh);
+      const bf16x8 xl = ld16(X + (32 * rb + l32) * 32 + 16 + 8 * lh);
+#pragma unroll
+      for (int ob = 0; ob < 2; ++ob) {
+        acc[ob][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ob][0], xh, b16[ob], 0, 0, 0);
+        acc[ob][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ob][1], xl, acc[ob][rb], 0, 0, 0);
+        acc[ob][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ob][2], xh, acc[ob][rb], 0, 0, 0);
       }
     }
   }
-  store_act(acc, H, o0, lr, lq);
+  store_act(acc, H, o0, l32, lh);
 
   // ---- hidden layers 1..n_hidden: 256 x 256 ----------------------------------------
   for (int l = 1; l <= P.n_hidden; ++l) {
-    load_bias(b4, P.bs + l * kW, o0, lq);
+    load_bias(b16, P.bs + l * kW, o0, lh);
     __syncthreads();  // H complete
-    hidden_mfma(acc, P.w[l], H, b4, o0, lr, lq);
+    hidden_mfma(acc, P.wsw + static_cast<size_t>(l - 1) * kW * kW, H, b16, o0, lane, l32, lh);
     __syncthreads();  // every wave has read H
-    store_act(acc, H, o0, lr, lq);
+    store_act(acc, H, o0, l32, lh);
   }
   __syncthreads();
 
@@ -377,6 +401,22 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
 struct BiasPtrs {
   const float* p[kMaxLayers];
 };
+
+// Hidden layers 1..n_hidden (row-major [out][in] BF16) in fragment order:
+// layer j's fragment of feature block fb and K-step ks for lane l is the 8
+// weights W[32 fb + l % 32][16 ks + 8 (l / 32) .. + 7], stored at
+// out[j kW^2 + ((fb 16 + ks) 64 + l) 8].  Grid n_hidden x 8 x 16, 64 lanes.
+struct LayerPtrs {
+  const uint16_t* p[kMaxLayers];
+};
+
+__global__ void k_swizzle(const LayerPtrs W, uint16_t* __restrict__ out) {
+  const int b = blockIdx.x, l = threadIdx.x;
+  const int j = b / 128, fb = (b / 16) % 8, ks = b % 16;
+  const uint16_t* src = W.p[j] + (32 * fb + (l & 31)) * kW + 16 * ks + 8 * (l >> 5);
+  *reinterpret_cast<uint4*>(out + static_cast<size_t>(j) * kW * kW + ((fb * 16 + ks) * 64 + l) * 8) =
+      *reinterpret_cast<const uint4*>(src);
+}
 
 __global__ void k_prep(const float* __restrict__ W0, int ld, const BiasPtrs biases, int n_bias,
                        uint16_t* __restrict__ w0cat, float* __restrict__ bs) {
@@ -592,19 +632,22 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
   const int n_bias = v.L - 1;  // SiLU layers 0..n_hidden
   const size_t w0_bytes = 256 * 64 * sizeof(uint16_t);
   const size_t bs_bytes = static_cast<size_t>(kMaxLayers) * kW * sizeof(float);
-  const size_t bytes = w0_bytes + bs_bytes + hc.size() * sizeof(CamDev);
+  const int n_hidden = v.L - 2;
+  const size_t wsw_bytes = static_cast<size_t>(n_hidden) * kW * kW * sizeof(uint16_t);
+  const size_t bytes = w0_bytes + bs_bytes + wsw_bytes + hc.size() * sizeof(CamDev);
   if (*v.render_cap < bytes) {
     if (*v.render_buf) (void)hipFree(*v.render_buf);
     *v.render_buf = nullptr;
     *v.render_cap = 0;
-    const size_t cap = std::max<size_t>(bytes, w0_bytes + bs_bytes + 64 * sizeof(CamDev));
+    const size_t cap = std::max<size_t>(bytes, w0_bytes + bs_bytes + wsw_bytes + 64 * sizeof(CamDev));
     if (hipMalloc(v.render_buf, cap) != hipSuccess) return fail("bh_surrogate_render: hipMalloc failed");
     *v.render_cap = cap;
   }
   char* scratch = static_cast<char*>(*v.render_buf);
   uint16_t* d_w0 = reinterpret_cast<uint16_t*>(scratch);
   float* d_bs = reinterpret_cast<float*>(scratch + w0_bytes);
-  CamDev* d_cams = reinterpret_cast<CamDev*>(scratch + w0_bytes + bs_bytes);
+  uint16_t* d_wsw = reinterpret_cast<uint16_t*>(scratch + w0_bytes + bs_bytes);
+  CamDev* d_cams = reinterpret_cast<CamDev*>(scratch + w0_bytes + bs_bytes + wsw_bytes);
   // Pageable source: the runtime stages it before returning.
   if (hipMemcpyAsync(d_cams, hc.data(), hc.size() * sizeof(CamDev), hipMemcpyHostToDevice, st) != hipSuccess)
     return fail("bh_surrogate_render: camera upload failed");
@@ -612,6 +655,12 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
   std::memset(&bp, 0, sizeof(bp));
   for (int l = 0; l < n_bias; ++l) bp.p[l] = v.p32 + v.b_off[l];
   hipLaunchKernelGGL(k_prep, dim3(256), dim3(64), 0, st, v.p32 + v.w_off[0], BH_SUR_IN_PAD, bp, n_bias, d_w0, d_bs);
+  if (n_hidden > 0) {
+    LayerPtrs lp;
+    std::memset(&lp, 0, sizeof(lp));
+    for (int l = 1; l <= n_hidden; ++l) lp.p[l - 1] = v.pbf + v.w_off[l];
+    hipLaunchKernelGGL(k_swizzle, dim3(n_hidden * 8 * 16), dim3(64), 0, st, lp, d_wsw);
+  }
   InferParams P;
   std::memset(&P, 0, sizeof(P));
   P.cams = d_cams;
@@ -623,6 +672,7 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
     P.w[l] = v.pbf + v.w_off[l];
     P.b[l] = v.p32 + v.b_off[l];
   }
+  P.wsw = d_wsw;
   P.out = out;
   P.fb_ids = reinterpret_cast<long long*>(fallback_ids);
   P.fb_count = reinterpret_cast<unsigned long long*>(fallback_count);
