@@ -29,7 +29,7 @@
 namespace bhk {
 int fail(const std::string& msg);
 
-__global__ void __launch_bounds__(256) k_mino_disk(const MinoParams P) {
+__global__ void __launch_bounds__(256, 4) k_mino_disk(const MinoParams P) {
   const long long gid = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   bool fb = false;
   if (gid < P.n_total) {

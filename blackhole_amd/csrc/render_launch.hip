@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/bh_kernel_launch.h"
+#include "grmhd_device.hpp"
 #include "trace_common.hpp"
 
 namespace bhk {
@@ -31,11 +32,19 @@ struct ShadeParams {
   double r_isco, r_out;
   float lit, bg;
   int redshift, bg_on;
+  // GRMHD modulation (use_luts and a volume in the grmhd slot): disk colour x rho u
+  unsigned long long grmhd, grmhd_right;
+  float grmhd_alpha, grmhd_r_min, grmhd_r_max;
+  int grmhd_on;
+  double len;  // launch length unit per M (rs / 2)
 };
 
 // Framebuffer colour of one record: disk -> blackbody tint of the observed
 // temperature g F^(1/4) times the bolometric brightness 2 F g^4 (g^4 only
-// with redshift on), horizon -> opaque black, otherwise the background.
+// with redshift on), scaled by max(rho, 0) max(u, 0) of the GRMHD volume at
+// the hit point (r at the crossing, theta = pi/2, phi at the end of the
+// crossing step) when GRMHD is on (d_shade_hit, device_physics.cuh:1842-1868);
+// horizon -> opaque black, otherwise the background.
 __global__ void k_shade(const ShadeParams p) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
@@ -52,6 +61,16 @@ __global__ void k_shade(const ShadeParams p) {
     const float tg = fminf(fmaxf(0.25f + 0.75f * T, 0.0f), 1.0f);
     const float tb = fminf(T * T, 1.0f);
     c = make_float4(lum, lum * tg, lum * tb, 1.0f);
+    if (p.grmhd_on) {
+      const float rr = static_cast<float>(rc * p.len);
+      const float ph = static_cast<float>(r.phi_exit);
+      const float4 q = grmhd_sample(p.grmhd, p.grmhd_right, p.grmhd_alpha, p.grmhd_r_min, p.grmhd_r_max,
+                                    rr * cosf(ph), rr * sinf(ph), 0.0f);
+      const float scale = fmaxf(q.x, 0.0f) * fmaxf(q.y, 0.0f);
+      c.x *= scale;
+      c.y *= scale;
+      c.z *= scale;
+    }
   } else if (r.hit_class == BH_HIT_HORIZON) {
     c = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   } else {
@@ -238,6 +257,16 @@ int bh_launch_geodesic_kernel(void* framebuffer, const struct BH_LaunchParams* p
   sp.bg = params->background_intensity;
   sp.redshift = params->redshift_enabled;
   sp.bg_on = params->background_enabled;
+  {
+    std::lock_guard<std::mutex> lk(g_tex_mu);
+    sp.grmhd = grmhd_volume_known(g_tex[5]) ? g_tex[5] : 0ull;
+    sp.grmhd_right = grmhd_volume_known(g_tex[7]) ? g_tex[7] : 0ull;
+  }
+  sp.grmhd_on = params->use_luts != 0 && sp.grmhd != 0ull;
+  sp.grmhd_alpha = params->grmhd_alpha;
+  sp.grmhd_r_min = params->grmhd_r_min;
+  sp.grmhd_r_max = params->grmhd_r_max;
+  sp.len = 0.5 * static_cast<double>(params->rs);
   const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
   hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream), sp);
   const hipError_t e = hipGetLastError();

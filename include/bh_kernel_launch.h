@@ -35,8 +35,14 @@
  *                the g^4 otherwise), alpha = 1;
  *   horizon   -> (0, 0, 0, 1);
  *   escaped   -> background_intensity * background_enabled in rgb, alpha 0.
+ * GRMHD modulation (d_shade_hit): with use_luts set and a volume from
+ * bh_grmhd_texture_create (bh_grmhd.h) in the grmhd slot of
+ * bh_upload_lut_textures, disk colours are scaled by max(rho, 0) max(u, 0)
+ * sampled at the hit point (u = (r - grmhd_r_min) / (grmhd_r_max -
+ * grmhd_r_min), v = theta / pi, w = phi / 2 pi; r clamped, theta mirrored,
+ * phi wrapped), blended with the grmhd_right volume by grmhd_alpha.
  * Cosmetic layers of the reference renderer (star field, glow, wiregrid,
- * debug exports, Stokes, GRMHD texture sampling) are accepted and ignored.
+ * debug exports, Stokes) are accepted and ignored.
  *
  * Returns 0 / -1 (bh_last_error); asynchronous on `stream` (NULL = default).
  */
@@ -154,10 +160,12 @@ int bh_launch_geodesic_kernel(void* framebuffer,
  * FP64 islands diverge within waves, so the FP64 form is the faster one. */
 int bh_select_kernel_variant(void);
 
-/* src/cuda/kernel_launch.h:176-183: texture handles (hipTextureObject_t as
- * u64, 0 = absent).  Stored process-wide (as the reference's constant-memory
- * slots) and reported back by bh_lut_texture_handle; the tracer computes the
- * disk flux and redshift in closed form and does not sample them. */
+/* src/cuda/kernel_launch.h:176-183: texture handles (u64, 0 = absent),
+ * stored process-wide (as the reference's constant-memory slots) and reported
+ * back by bh_lut_texture_handle.  The tracer computes the disk flux and
+ * redshift in closed form and does not sample the LUT slots; the grmhd and
+ * grmhd_right slots take volumes from bh_grmhd_texture_create (gfx950 has no
+ * image-sampling hardware; other handles are ignored). */
 void bh_upload_lut_textures(unsigned long long emissivity,
                             unsigned long long redshift,
                             unsigned long long spectral,

@@ -113,13 +113,15 @@ def test_struct_layouts_match_the_c_headers(tmp_path):
     import subprocess
     from blackhole_amd import abi
     from blackhole_amd import surrogate as S
+    from blackhole_amd import grmhd as G
     structs = {
         "BH_Camera": abi.BH_Camera, "BH_TraceOptions": abi.BH_TraceOptions, "BH_Volume": abi.BH_Volume,
         "BH_VolumeSet": abi.BH_VolumeSet, "BH_TuneResult": abi.BH_TuneResult, "BH_RaySpec": abi.BH_RaySpec,
         "BH_SurrogateRenderParams": S.RenderParams, "BH_SurrogateConfig": S.SurrogateConfig,
-        "BH_TorusParams": abi.BH_TorusParams,
+        "BH_TorusParams": abi.BH_TorusParams, "BH_GrmhdPacked": G.BH_GrmhdPacked,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bhk_trace.h"', '#include "bhk_surrogate.h"',
+             '#include "bh_grmhd.h"',
              "int main(void) {"]
     for name, cls in structs.items():
         lines.append(f'  printf("{name} sizeof %zu\\n", sizeof({name}));')
