@@ -96,6 +96,10 @@ def lib():
     L.bh_trace_disk_mino.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.BH_TraceOptions), C.c_void_p, C.c_int,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
     L.bh_trace_disk_mino.restype = C.c_int
+    L.bh_trace_image_plane_pol.argtypes = [
+        C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions), C.c_void_p, C.c_int,
+        C.POINTER(abi.BH_Volume), C.POINTER(abi.BH_PolParams), C.c_void_p, C.c_void_p, C.c_void_p]
+    L.bh_trace_image_plane_pol.restype = C.c_int
     # renderer backend (bh_backend.h)
     L.bhCudaInit.argtypes = [C.c_uint, C.c_int, C.c_int]
     L.bhCudaInit.restype = C.c_void_p

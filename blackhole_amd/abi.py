@@ -97,6 +97,16 @@ class BH_VolumeSet(C.Structure):
     ]
 
 
+class BH_PolParams(C.Structure):
+    """Polarised transfer parameters (bhk_trace.h BH_PolParams, 32 bytes)."""
+    _fields_ = [
+        ("evpa_b", C.c_double),
+        ("lin_pol_frac", C.c_double),
+        ("faraday_per_ne", C.c_double),
+        ("pad0", C.c_double),
+    ]
+
+
 class BH_TuneResult(C.Structure):
     _fields_ = [
         ("blocks_per_cu", C.c_int32),
@@ -171,6 +181,7 @@ RECORD_DTYPE = np.dtype(
 
 assert C.sizeof(BH_PixelRecord) == 128
 assert C.sizeof(BH_Camera) == 72
+assert C.sizeof(BH_PolParams) == 32
 
 
 # ---------------------------------------------------------------------------

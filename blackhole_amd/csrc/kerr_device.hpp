@@ -520,11 +520,15 @@ struct VolDev {
   double r_min, r_max, r_span, sup_r_lo, sup_r_hi, sup_u_max, ell;
 };
 
+// rho (optional): the Faraday rotation rate per unit affine parameter,
+// faraday_per_ne n_e L / g (rad; zero outside the support).
 __device__ __forceinline__ bool volume_point(const VolDev& v, const RayC& c, double r, double u,
                                              double phi, double nu_obs, double L, double* a,
-                                             double* e) {
+                                             double* e, double faraday_per_ne = 0.0,
+                                             double* rho = nullptr) {
   *a = 0.0;
   *e = 0.0;
+  if (rho) *rho = 0.0;
   if (!(r >= v.r_min && r <= v.r_max && r >= v.sup_r_lo && r <= v.sup_r_hi &&
         __builtin_fabs(u) <= v.sup_u_max))
     return false;
@@ -534,6 +538,7 @@ __device__ __forceinline__ bool volume_point(const VolDev& v, const RayC& c, dou
   if (!(ne > 0.0 && th > 0.0 && B > 0.0 && k2r > 0.0)) return true;
   const double g = fluid_g(r, u, c.a, c.xi, v.ell);
   if (!(g > 0.0)) return true;
+  if (rho) *rho = faraday_per_ne * ne * L / g;
   const double nu_f = nu_obs / g;
   const double tempK = th * (kMElectron * kC * kC / kKB);
   const double Bnu = planck(nu_f, tempK);
@@ -572,6 +577,55 @@ __device__ __forceinline__ void rt_segment(RtAcc& q, double h, double a1, double
   q.a0 = a1;
   q.e0 = e1;
   q.s0 = s1;
+}
+
+// Polarised transfer beside the scalar one: linear (Q, U) under total
+// absorption and Faraday rotation (the simplified Mueller matrix of
+// stokesStep, src/physics/stokes_transport.h:260-369; V has no source and
+// stays zero), accumulated from the observer backwards like the scalar I.
+// Rotations of the (Q, U) plane commute, so the state is the observed
+// (Q, U), the transmittance (the scalar's Tr) and the cumulative Faraday
+// angle Phi as (cos, sin).  Over one step the source (j_Q, j_U) = -frac e
+// (cos 2 chi, sin 2 chi) with the trapezoid-averaged emission e,
+// absorption A and Faraday rate R per affine parameter contributes, at the
+// step's observer-side end, (j_Q + i j_U) (1 - e^{-w}) / z, z = A - i R,
+// w = z dl -- the exact integral of the emission attenuated and rotated
+// within the step -- and reaches the observer rotated by Phi and scaled by
+// Tr.  Must run before rt_segment updates Tr and the step's left values.
+struct PolAcc {
+  double q, u, c, s, r0;
+};
+
+__device__ __forceinline__ void pol_segment(PolAcc& p, const RtAcc& q, double h, double a1, double e1,
+                                            double r1, double s1, double c2, double s2, double frac) {
+  const double esum = q.e0 + e1;
+  const double rsum = p.r0 + r1;
+  if (esum > 0.0 || rsum != 0.0) {
+    const double dl = 0.5 * h * (q.s0 + s1);
+    const double A = 0.5 * (q.a0 + a1), R = 0.5 * rsum, J = 0.5 * esum;
+    const double x = A * dl, z = R * dl;
+    double sz, cz;
+    sincos(z, &sz, &cz);
+    double ic, is;
+    if (x * x + z * z < 1e-6) {  // (1 - e^{-w}) / z to third order in w
+      ic = dl * (1.0 - 0.5 * x + (x * x - z * z) * (1.0 / 6.0));
+      is = dl * (0.5 * z - x * z * (1.0 / 3.0));
+    } else {
+      const double E = exp(-x);
+      const double D = A * A + R * R;
+      const double omc = 1.0 - E * cz;
+      ic = (A * omc + R * E * sz) / D;
+      is = (R * omc - A * E * sz) / D;
+    }
+    const double jq = -frac * J * c2, ju = -frac * J * s2;
+    const double qe = jq * ic - ju * is, ue = jq * is + ju * ic;
+    p.q = __builtin_fma(q.Tr, p.c * qe - p.s * ue, p.q);
+    p.u = __builtin_fma(q.Tr, p.s * qe + p.c * ue, p.u);
+    const double nc = p.c * cz - p.s * sz, ns = p.s * cz + p.c * sz;
+    p.c = nc;
+    p.s = ns;
+  }
+  p.r0 = r1;
 }
 
 }  // namespace bhk

@@ -39,6 +39,7 @@ struct Lane {
   double y[kN], k1[kN];
   double h;
   RtAcc rt;
+  PolAcc pol;  // polarised transfer (POL kernels)
   double r_stop, r_isco;
   long long gid;
   int cam;
@@ -76,18 +77,23 @@ __device__ __forceinline__ bool lane_setup(Lane& L, const DevParams& P, long lon
 
 // Volume transfer over the accepted step of length h ending at y.  MV: the
 // lane's camera samples its own volume of a volume set (same geometry and
-// support as P.vol, own cells and fluid angular momentum).
-template <bool MV>
+// support as P.vol, own cells and fluid angular momentum).  POL: the linear
+// polarisation is carried beside I (pol_segment).
+template <bool MV, bool POL>
 __device__ __forceinline__ void volume_segment(Lane& L, const DevParams& P, const double* y,
                                                double h) {
-  double a1, e1;
+  double a1, e1, r1 = 0.0;
   VolDev v = P.vol;
   if (MV) {
     v.cells = P.buckets[L.vb].cells;
     v.ell = P.buckets[L.vb].ell;
   }
-  if (volume_point(v, L.c, y[0], y[1], y[2], P.nu_obs, P.length_unit_cm, &a1, &e1)) L.nsamp++;
-  rt_segment(L.rt, h, a1, e1, affine_rate(L.c, y[0], y[1]));
+  if (volume_point(v, L.c, y[0], y[1], y[2], P.nu_obs, P.length_unit_cm, &a1, &e1, P.faraday_per_ne,
+                   POL ? &r1 : nullptr))
+    L.nsamp++;
+  const double s1 = affine_rate(L.c, y[0], y[1]);
+  if (POL) pol_segment(L.pol, L.rt, h, a1, e1, r1, s1, P.pol_c2, P.pol_s2, P.pol_frac);
+  rt_segment(L.rt, h, a1, e1, s1);
 }
 
 // Equatorial crossing between (y0, f0) and (y1, f1); true if the disk ends the ray.
@@ -153,7 +159,7 @@ __device__ __forceinline__ void path_point(const DevParams& P, Lane& L) {
 }
 
 // One RK4 step; returns the terminal status (0 = continue).
-template <bool VOL, bool MV, bool PATH>
+template <bool VOL, bool MV, bool PATH, bool POL>
 __device__ __forceinline__ int step_rk4(Lane& L, const DevParams& P) {
   if (L.steps >= P.max_steps) return BH_HIT_MAX_STEPS;
   const double h = P.h_fixed;
@@ -163,7 +169,7 @@ __device__ __forceinline__ int step_rk4(Lane& L, const DevParams& P) {
   project_radial(L.c, yn, fn);
   L.steps++;
   L.attempts++;
-  if (VOL) volume_segment<MV>(L, P, yn, h);
+  if (VOL) volume_segment<MV, POL>(L, P, yn, h);
   const bool hit = crossing(L, P, L.y, L.k1, yn, fn, h);
 #pragma unroll
   for (int i = 0; i < kN; ++i) {
@@ -176,7 +182,7 @@ __device__ __forceinline__ int step_rk4(Lane& L, const DevParams& P) {
 }
 
 // One DP45 attempt; returns the terminal status (0 = continue).
-template <bool VOL, bool MV, bool PATH>
+template <bool VOL, bool MV, bool PATH, bool POL>
 __device__ __forceinline__ int step_dp45(Lane& L, const DevParams& P) {
   if (L.steps >= P.max_steps || L.attempts >= P.max_attempts) return BH_HIT_MAX_STEPS;
   double cap = P.h_max;
@@ -189,7 +195,7 @@ __device__ __forceinline__ int step_dp45(Lane& L, const DevParams& P) {
     const double fac = dp45_factor(err, true, L.last_rej);
     project_radial(L.c, y5, k7);
     L.steps++;
-    if (VOL) volume_segment<MV>(L, P, y5, L.h);
+    if (VOL) volume_segment<MV, POL>(L, P, y5, L.h);
     const bool hit = crossing(L, P, L.y, L.k1, y5, k7, L.h);
 #pragma unroll
     for (int i = 0; i < kN; ++i) {
@@ -210,7 +216,7 @@ __device__ __forceinline__ int step_dp45(Lane& L, const DevParams& P) {
 // Two workgroups per CU: the volume variants would otherwise take a few
 // registers past the 256 that two waves per SIMD allow (the handful of
 // spilled values cost less than the lost latency hiding).
-template <int INTEG, bool VOL, bool MV, bool PATH>
+template <int INTEG, bool VOL, bool MV, bool PATH, bool POL = false>
 __global__ void __launch_bounds__(kBlock, 2) k_trace_fp64(const DevParams* __restrict__ Pp) {
   const DevParams& P = *Pp;
   Lane L;
@@ -224,9 +230,12 @@ __global__ void __launch_bounds__(kBlock, 2) k_trace_fp64(const DevParams* __res
       if (id < n_queue) {
         if (!lane_setup<VOL>(L, P, id)) {
           L.gid = -1;
-        } else if (PATH) {
-          L.npts = 0;
-          path_point(P, L);
+        } else {
+          if (PATH) {
+            L.npts = 0;
+            path_point(P, L);
+          }
+          if (POL) L.pol = PolAcc{0.0, 0.0, 1.0, 0.0, 0.0};
         }
       } else {
         done = true;
@@ -235,9 +244,11 @@ __global__ void __launch_bounds__(kBlock, 2) k_trace_fp64(const DevParams* __res
     if (__ballot(!done) == 0ull) break;
     if (L.gid >= 0) {
       for (int b = 0; b < P.batch; ++b) {
-        const int st = (INTEG == BH_INTEGRATOR_DP45) ? step_dp45<VOL, MV, PATH>(L, P) : step_rk4<VOL, MV, PATH>(L, P);
+        const int st = (INTEG == BH_INTEGRATOR_DP45) ? step_dp45<VOL, MV, PATH, POL>(L, P)
+                                                      : step_rk4<VOL, MV, PATH, POL>(L, P);
         if (st) {
           write_record(L, P, st);
+          if (POL) P.stokes[L.gid] = make_double4(L.rt.I, L.pol.q, L.pol.u, 0.0);
           if (PATH) P.path_count[L.gid] = L.npts;
           L.gid = -1;
           break;
@@ -270,16 +281,18 @@ struct KernelKey {
   int integ, prec;
   bool vol, multi;
   bool path = false;  // polyline recording (DP45, FP64, no volume only)
+  bool pol = false;   // polarised transfer (FP64, one volume)
 };
 
-template <int I, bool V, bool M, bool PT = false>
+template <int I, bool V, bool M, bool PT = false, bool PO = false>
 int max_blocks_per_cu() {
   static int cached[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
   if (cached[dev] > 0) return cached[dev];
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_fp64<I, V, M, PT>, kBlock, 0) != hipSuccess) return -1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_fp64<I, V, M, PT, PO>, kBlock, 0) != hipSuccess)
+    return -1;
   cached[dev] = std::max(1, nb);
   return cached[dev];
 }
@@ -298,6 +311,9 @@ int cu_count() {
 int occupancy_limit(const KernelKey& k) {
   if (k.prec == BH_PRECISION_FP32) return occupancy_limit_fp32(k.integ, k.vol);
   if (k.path) return max_blocks_per_cu<BH_INTEGRATOR_DP45, false, false, true>();
+  if (k.pol)
+    return k.integ == BH_INTEGRATOR_DP45 ? max_blocks_per_cu<BH_INTEGRATOR_DP45, true, false, false, true>()
+                                         : max_blocks_per_cu<BH_INTEGRATOR_RK4, true, false, false, true>();
   if (k.integ == BH_INTEGRATOR_DP45) {
     if (k.multi) return max_blocks_per_cu<BH_INTEGRATOR_DP45, true, true>();
     return k.vol ? max_blocks_per_cu<BH_INTEGRATOR_DP45, true, false>()
@@ -313,6 +329,13 @@ hipError_t launch_kernel(const KernelKey& k, const DevParams* dp, hipStream_t s,
   hipLaunchKernelGGL((k_trace_fp64<I, V, M, false>), dim3(grid), dim3(kBlock), 0, s, dp)
   if (k.path) {
     hipLaunchKernelGGL((k_trace_fp64<BH_INTEGRATOR_DP45, false, false, true>), dim3(grid), dim3(kBlock), 0, s, dp);
+  } else if (k.pol) {
+    if (k.integ == BH_INTEGRATOR_DP45)
+      hipLaunchKernelGGL((k_trace_fp64<BH_INTEGRATOR_DP45, true, false, false, true>), dim3(grid), dim3(kBlock), 0, s,
+                         dp);
+    else
+      hipLaunchKernelGGL((k_trace_fp64<BH_INTEGRATOR_RK4, true, false, false, true>), dim3(grid), dim3(kBlock), 0, s,
+                         dp);
   } else if (k.integ == BH_INTEGRATOR_DP45) {
     if (k.multi) BHK_LAUNCH(BH_INTEGRATOR_DP45, true, true);
     else if (k.vol) BHK_LAUNCH(BH_INTEGRATOR_DP45, true, false);
@@ -776,6 +799,30 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
                          BH_PixelRecord* out, void* stream) {
   return trace_launch(cams, n_cams, opts, disk_luts, disk_lut_n, vol, nullptr, nullptr, 0, nullptr, nullptr, 0, out,
                       stream, false, "bh_trace_image_plane");
+}
+
+int bh_trace_image_plane_pol(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts, const double* disk_luts,
+                             int disk_lut_n, const BH_Volume* vol, const BH_PolParams* pp, BH_PixelRecord* out,
+                             double* stokes, void* stream) {
+  const char* who = "bh_trace_image_plane_pol";
+  if (!pp || !stokes) return fail(std::string(who) + ": NULL polarisation parameters or Stokes buffer");
+  if (!opts || !opts->volume_enabled || !vol)
+    return fail(std::string(who) + ": polarised transfer needs volume_enabled and a volume");
+  if (opts->precision != BH_PRECISION_FP64) return fail(std::string(who) + ": the polarised form runs in FP64");
+  if (!(pp->lin_pol_frac >= 0.0 && pp->lin_pol_frac <= 1.0) || !std::isfinite(pp->evpa_b) ||
+      !std::isfinite(pp->faraday_per_ne))
+    return fail(std::string(who) + ": lin_pol_frac must be in [0, 1], evpa_b and faraday_per_ne finite");
+  LaunchImage im;
+  if (build_launch(cams, n_cams, opts, disk_luts, disk_lut_n, vol, nullptr, nullptr, 0, nullptr, nullptr, 0, out,
+                   false, who, &im) != 0)
+    return -1;
+  im.P.stokes = reinterpret_cast<double4*>(stokes);
+  im.P.pol_c2 = std::cos(2.0 * pp->evpa_b);
+  im.P.pol_s2 = std::sin(2.0 * pp->evpa_b);
+  im.P.pol_frac = pp->lin_pol_frac;
+  im.P.faraday_per_ne = pp->faraday_per_ne;
+  im.key.pol = true;
+  return launch_on_stream(im, stream, who);
 }
 
 int bh_trace_image_plane_multi(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,

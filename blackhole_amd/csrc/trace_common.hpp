@@ -58,6 +58,12 @@ struct DevParams {
   // x_M = Cx / (g |B| Theta_e^2),  j L = CjL n_e |B| I_M(x_M) / k2r,
   // disk: x = Cd / (g F^(1/4)),  I = SB / expm1(x).
   float CT, SB, Cx, CjL, Cd, pad32;
+  // Polarised transfer (POL kernels): (I, Q, U, V) per record into stokes;
+  // the volume's emission is linearly polarised at pol_frac along the sky
+  // angle chi (pol_c2 = cos 2 chi, pol_s2 = sin 2 chi) and its Faraday rate
+  // is faraday_per_ne n_e per fluid-frame cm.
+  double4* stokes;
+  double pol_c2, pol_s2, pol_frac, faraday_per_ne;
 };
 
 __device__ __forceinline__ int find_camera(const CamDev* cams, int n, long long gid) {

@@ -151,6 +151,30 @@ def launch(sc: DeviceScene, out, stream=None) -> None:
     check(rc, "bh_trace_image_plane")
 
 
+def launch_pol(sc: DeviceScene, out, stokes, evpa_b: float, lin_pol_frac: float,
+               faraday_per_ne: float, stream=None) -> None:
+    """launch() with the polarised transfer (FP64, one volume): the same
+    records into out and (I, Q, U, V) per record into stokes (float64 cuda
+    [n, 4])."""
+    torch = _torch()
+    assert out.dtype == torch.uint8 and out.is_cuda and out.shape[1] == 128
+    assert out.shape[0] >= sc.n_records
+    assert stokes.dtype == torch.float64 and stokes.is_cuda and stokes.is_contiguous()
+    assert stokes.shape[0] >= sc.n_records and stokes.shape[1] == 4
+    if sc.vol is None:
+        raise ValueError("polarised transfer needs a volume scene")
+    pp = abi.BH_PolParams(evpa_b, lin_pol_frac, faraday_per_ne, 0.0)
+    cam_arr = (abi.BH_Camera * len(sc.cams))(*sc.cams)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    rc = lib().bh_trace_image_plane_pol(
+        cam_arr, len(sc.cams), C.byref(sc.opts),
+        C.c_void_p(sc.luts.data_ptr()) if sc.luts is not None else None, sc.lut_n,
+        C.byref(sc.vol), C.byref(pp), C.c_void_p(out.data_ptr()), C.c_void_p(stokes.data_ptr()),
+        C.c_void_p(s.cuda_stream),
+    )
+    check(rc, "bh_trace_image_plane_pol")
+
+
 def launch_ids(sc: DeviceScene, ids, count, out, stream=None, compact: bool = False) -> None:
     """Asynchronous trace of the pixels listed in ids (int64 cuda) with the
     list length in count (int64/uint64 cuda, one element), records to out[id]

@@ -337,6 +337,29 @@ int bh_trace_rays_host(const BH_RaySpec* rays, int64_t n, const BH_TraceOptions*
  * caller must have synchronised the stream. */
 float bh_last_kernel_ms(void);
 
+/* Polarised transfer (src/physics/stokes_transport.h; the device path of
+ * d_stokes_step / d_trace_geodesic_stokes, device_physics.cuh:2185-2459): the
+ * volume's thermal synchrotron emission is linearly polarised at
+ * lin_pol_frac along the sky angle evpa_b (j_Q + i j_U = -lin_pol_frac j_I
+ * e^{2 i evpa_b}), and the plasma rotates (Q, U) at the Faraday rate
+ * rho_V = faraday_per_ne n_e [rad per fluid-frame cm]; total absorption as
+ * for I, no dichroism, no conversion (the simplified Mueller matrix of
+ * stokesStep), so V stays 0.  The thin disk's emission is unpolarised. */
+typedef struct BH_PolParams {
+    double evpa_b;          /* polarisation angle chi of the emission [rad]      */
+    double lin_pol_frac;    /* linear polarisation fraction of j (0.75: thermal) */
+    double faraday_per_ne;  /* rho_V / n_e [rad cm^2]                            */
+    double pad0;
+} BH_PolParams; /* 32 bytes */
+
+/* bh_trace_image_plane with the polarised transfer: the same records (bit for
+ * bit) plus stokes[4 * record] = (I, Q, U, V) at the observer (device double
+ * per record).  FP64, volume_enabled, one volume. */
+int bh_trace_image_plane_pol(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+                             const double* disk_luts, int disk_lut_n, const BH_Volume* vol,
+                             const BH_PolParams* pol, BH_PixelRecord* out, double* stokes,
+                             void* stream);
+
 /* Thin-disk records without step integration: the equatorial crossings of
  * every pixel's geodesic from the Mino-time integrals of motion (polar
  * elliptic integrals in closed form, the radial integral by adaptive

@@ -627,10 +627,12 @@ inline T planck(T nu, T tempK) {
 // the support (no sample taken); a = e = 0 where the gas is empty.
 template <class T>
 inline bool volume_point(const VolumeHost& V, const RayConst<T>& c, T r, T u, T phi,
-                         double nu_obs, double L, T* a, T* e) {
+                         double nu_obs, double L, T* a, T* e, double faraday_per_ne = 0.0,
+                         T* rho = nullptr) {
   const BH_Volume& v = V.v;
   *a = T(0.0);
   *e = T(0.0);
+  if (rho) *rho = T(0.0);
   if (!(val_(r) >= v.r_min && val_(r) <= v.r_max && val_(r) >= v.sup_r_lo &&
         val_(r) <= v.sup_r_hi && std::fabs(val_(u)) <= v.sup_u_max))
     return false;
@@ -640,6 +642,7 @@ inline bool volume_point(const VolumeHost& V, const RayConst<T>& c, T r, T u, T 
   if (!(val_(ne) > 0.0 && val_(th) > 0.0 && val_(B) > 0.0 && val_(k2r) > 0.0)) return true;
   const T g = fluid_g(r, u, c.a, c.xi, T(v.ell));
   if (!(val_(g) > 0.0)) return true;
+  if (rho) *rho = T(faraday_per_ne) * ne * T(L) / g;  // Faraday rate per affine parameter
   const T nu_f = T(nu_obs) / g;
   const T tempK = th * T(kMElectron * kC * kC / kKB);
   const T Bnu = planck(nu_f, tempK);
@@ -685,6 +688,48 @@ inline void rt_segment(RtAcc<T>& q, T h, T a1, T e1, T s1) {
   q.a0 = a1;
   q.e0 = e1;
   q.s0 = s1;
+}
+
+// Polarised transfer beside the scalar one (simplified Mueller matrix of
+// stokesStep, src/physics/stokes_transport.h:260-369): the observed (Q, U)
+// gains, per step, the source (j_Q + i j_U) = -frac e e^{2 i chi} integrated
+// exactly under absorption A and Faraday rate R within the step,
+// (1 - e^{-w}) / z with z = A - i R, w = z dl, rotated by the Faraday angle
+// Phi accumulated between the step and the observer and scaled by the
+// transmittance Tr before the step (called before rt_segment).
+struct PolAcc {
+  double q, u, c, s, r0;
+};
+
+inline void pol_segment(PolAcc& p, const RtAcc<double>& q, double h, double a1, double e1, double r1,
+                        double s1, double c2, double s2, double frac) {
+  const double esum = q.e0 + e1;
+  const double rsum = p.r0 + r1;
+  if (esum > 0.0 || rsum != 0.0) {
+    const double dl = 0.5 * h * (q.s0 + s1);
+    const double A = 0.5 * (q.a0 + a1), R = 0.5 * rsum, J = 0.5 * esum;
+    const double x = A * dl, z = R * dl;
+    const double sz = std::sin(z), cz = std::cos(z);
+    double ic, is;
+    if (x * x + z * z < 1e-6) {
+      ic = dl * (1.0 - 0.5 * x + (x * x - z * z) * (1.0 / 6.0));
+      is = dl * (0.5 * z - x * z * (1.0 / 3.0));
+    } else {
+      const double E = std::exp(-x);
+      const double D = A * A + R * R;
+      const double omc = 1.0 - E * cz;
+      ic = (A * omc + R * E * sz) / D;
+      is = (R * omc - A * E * sz) / D;
+    }
+    const double jq = -frac * J * c2, ju = -frac * J * s2;
+    const double qe = jq * ic - ju * is, ue = jq * is + ju * ic;
+    p.q = std::fma(q.Tr, p.c * qe - p.s * ue, p.q);
+    p.u = std::fma(q.Tr, p.s * qe + p.c * ue, p.u);
+    const double nc = p.c * cz - p.s * sz, ns = p.s * cz + p.c * sz;
+    p.c = nc;
+    p.s = ns;
+  }
+  p.r0 = r1;
 }
 
 }  // namespace orc

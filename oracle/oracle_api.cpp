@@ -29,6 +29,8 @@ struct Scene {
   int lut_n;
   bool vol_on;
   VolumeHost V;
+  bool pol_on = false;  // polarised transfer
+  double pol_c2 = 1.0, pol_s2 = 0.0, pol_frac = 0.0, faraday = 0.0;
 };
 
 inline bool in_support(const BH_Volume& v, double r, double u) {
@@ -41,15 +43,19 @@ struct RayAcc {
   int ncross = 0;
   int nsamples = 0;
   double cross_r[3] = {0, 0, 0}, cross_g[3] = {0, 0, 0};
+  PolAcc pol{0.0, 0.0, 1.0, 0.0, 0.0};
 };
 
 // Volume transfer over the accepted step of length h that ended at y.
 inline void volume_segment(const Scene& S, const RayConst<double>& c, const double* y, double h,
                            RayAcc& acc) {
-  double a1, e1;
-  if (volume_point(S.V, c, y[0], y[1], y[2], S.o->nu_obs, S.o->length_unit_cm, &a1, &e1))
+  double a1, e1, r1 = 0.0;
+  if (volume_point(S.V, c, y[0], y[1], y[2], S.o->nu_obs, S.o->length_unit_cm, &a1, &e1, S.faraday,
+                   S.pol_on ? &r1 : nullptr))
     acc.nsamples++;
-  rt_segment(acc.rt, h, a1, e1, affine_rate(c, y[0], y[1]));
+  const double s1 = affine_rate(c, y[0], y[1]);
+  if (S.pol_on) pol_segment(acc.pol, acc.rt, h, a1, e1, r1, s1, S.pol_c2, S.pol_s2, S.pol_frac);
+  rt_segment(acc.rt, h, a1, e1, s1);
 }
 
 // Returns true when the disk terminated the ray.
@@ -99,6 +105,15 @@ inline void finish(const RayConst<double>& c, const double* y, const RayAcc& acc
   rec->n_samples = acc.nsamples;
 }
 
+// (I, Q, U, V) at the observer; V = 0 (no Faraday conversion in the
+// simplified Mueller matrix).
+inline void write_stokes(const RayAcc& acc, double* st) {
+  st[0] = acc.rt.I;
+  st[1] = acc.pol.q;
+  st[2] = acc.pol.u;
+  st[3] = 0.0;
+}
+
 inline int terminate_check(const Scene& S, const CamConst& k, const double* y, const RayAcc& acc) {
   if (y[0] < k.r_stop) return BH_HIT_HORIZON;
   if (y[0] > S.o->r_escape) return BH_HIT_ESCAPED;
@@ -106,7 +121,8 @@ inline int terminate_check(const Scene& S, const CamConst& k, const double* y, c
   return 0;
 }
 
-void trace_rk4(const Scene& S, const CamConst& k, int64_t px, int64_t py, BH_PixelRecord* rec) {
+void trace_rk4(const Scene& S, const CamConst& k, int64_t px, int64_t py, BH_PixelRecord* rec,
+               double* st = nullptr) {
   double y[kN], k1[kN], yn[kN], fn[kN];
   const RayConst<double> c = setup_ray<double>(k, px, py, y);
   const double h = S.o->h_fixed;
@@ -135,9 +151,11 @@ void trace_rk4(const Scene& S, const CamConst& k, int64_t px, int64_t py, BH_Pix
     if (status) break;
   }
   finish(c, y, acc, status, steps, steps, rec);
+  if (st) write_stokes(acc, st);
 }
 
-void trace_dp45(const Scene& S, const CamConst& k, int64_t px, int64_t py, BH_PixelRecord* rec) {
+void trace_dp45(const Scene& S, const CamConst& k, int64_t px, int64_t py, BH_PixelRecord* rec,
+                double* st = nullptr) {
   double y[kN], k1[kN], y5[kN], k7[kN];
   const RayConst<double> c = setup_ray<double>(k, px, py, y);
   const BH_TraceOptions& o = *S.o;
@@ -179,6 +197,7 @@ void trace_dp45(const Scene& S, const CamConst& k, int64_t px, int64_t py, BH_Pi
     }
   }
   finish(c, y, acc, status, steps, attempts, rec);
+  if (st) write_stokes(acc, st);
 }
 
 int64_t camera_of(const std::vector<CamConst>& ks, int64_t gid) {
@@ -198,11 +217,19 @@ extern "C" {
 // Trace the rays with global record indices ids[0..n_ids) (or all rays when
 // ids is NULL) of the camera set; out[i] receives the record of ids[i].
 // volume_cells is a host array; nthreads <= 0 uses all cores.
-int orc_trace(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
-              const double* luts, int lut_n, const BH_Volume* vol, const float* volume_cells,
-              const int64_t* ids, int64_t n_ids, BH_PixelRecord* out, int nthreads) {
+static int trace_common(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+                        const double* luts, int lut_n, const BH_Volume* vol, const float* volume_cells,
+                        const int64_t* ids, int64_t n_ids, BH_PixelRecord* out, int nthreads,
+                        const double* pol, double* stokes) {
   if (!cams || n_cams <= 0 || !opts || !out) return -1;
   Scene S;
+  if (pol) {
+    S.pol_on = true;
+    S.pol_c2 = std::cos(2.0 * pol[0]);
+    S.pol_s2 = std::sin(2.0 * pol[0]);
+    S.pol_frac = pol[1];
+    S.faraday = pol[2];
+  }
   S.o = opts;
   S.luts = luts;
   S.lut_n = lut_n;
@@ -232,8 +259,10 @@ int orc_trace(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
         const int64_t ci = camera_of(ks, gid);
         const CamConst& k = ks[ci];
         const int64_t local = gid - k.offset;
+        double* st = stokes ? stokes + 4 * i : nullptr;
         if (local >= k.w * k.h) {  // gap between camera record ranges
           std::memset(&out[i], 0, sizeof(out[i]));
+          if (st) st[0] = st[1] = st[2] = st[3] = 0.0;
           continue;
         }
         const int64_t py = local / k.w, px = local % k.w;
@@ -241,9 +270,9 @@ int orc_trace(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
         Scene Sc = S;
         Sc.luts = lut;
         if (opts->integrator == BH_INTEGRATOR_DP45)
-          trace_dp45(Sc, k, px, py, &out[i]);
+          trace_dp45(Sc, k, px, py, &out[i], st);
         else
-          trace_rk4(Sc, k, px, py, &out[i]);
+          trace_rk4(Sc, k, px, py, &out[i], st);
       }
     }
   };
@@ -252,6 +281,42 @@ int orc_trace(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
   worker();
   for (auto& t : th) t.join();
   return 0;
+}
+
+int orc_trace(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+              const double* luts, int lut_n, const BH_Volume* vol, const float* volume_cells,
+              const int64_t* ids, int64_t n_ids, BH_PixelRecord* out, int nthreads) {
+  return trace_common(cams, n_cams, opts, luts, lut_n, vol, volume_cells, ids, n_ids, out, nthreads,
+                      nullptr, nullptr);
+}
+
+// orc_trace with the polarised transfer: pol = {evpa_b [rad], lin_pol_frac,
+// faraday_per_ne}; stokes[4 i] = (I, Q, U, V) of ray i at the observer.
+// Needs the volume (polarisation is carried only through volume segments).
+int orc_trace_pol(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+                  const double* luts, int lut_n, const BH_Volume* vol, const float* volume_cells,
+                  const double* pol, BH_PixelRecord* out, double* stokes, int nthreads) {
+  if (!pol || !stokes || !opts || !opts->volume_enabled) return -1;
+  return trace_common(cams, n_cams, opts, luts, lut_n, vol, volume_cells, nullptr, 0, out, nthreads, pol,
+                      stokes);
+}
+
+// One pol_segment step from a given state, for closed-form checks:
+// st_in = {q, u, c, s, r0, Tr, a0, e0, s0}, args = {h, a1, e1, r1, s1, c2, s2, frac};
+// st_out = {q, u, c, s, r0}.
+void orc_pol_segment(const double* st_in, const double* args, double* st_out) {
+  PolAcc p{st_in[0], st_in[1], st_in[2], st_in[3], st_in[4]};
+  RtAcc<double> q{};
+  q.Tr = st_in[5];
+  q.a0 = st_in[6];
+  q.e0 = st_in[7];
+  q.s0 = st_in[8];
+  pol_segment(p, q, args[0], args[1], args[2], args[3], args[4], args[5], args[6], args[7]);
+  st_out[0] = p.q;
+  st_out[1] = p.u;
+  st_out[2] = p.c;
+  st_out[3] = p.s;
+  st_out[4] = p.r0;
 }
 
 // Trace every pixel of a render launch (struct BH_LaunchParams) in FP64 as

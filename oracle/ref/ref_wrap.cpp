@@ -8,6 +8,7 @@
 #include "physics/iron_kline.h"
 #include "physics/kerr.h"
 #include "physics/rte_integrator.h"
+#include "physics/stokes_transport.h"
 #include "physics/verified/geodesic.hpp"
 
 namespace {
@@ -86,5 +87,45 @@ void ref_kerr_step_mino(double a, double e, double lz, double q, double* state6,
   state6[4] = s.signR;
   state6[5] = s.signTheta;
 }
+
+// stokes_transport.h: s4 / em4 = (I, Q, U, V) / (jI, jQ, jU, jV).
+void ref_stokes_step(const double* s4, const double* em4, double alpha_i, double rho_v, double ds,
+                     double* out4) {
+  const physics::StokesVector s{s4[0], s4[1], s4[2], s4[3]};
+  const physics::StokesEmission em{em4[0], em4[1], em4[2], em4[3]};
+  const physics::StokesVector o = physics::stokesStep(s, em, alpha_i, rho_v, ds);
+  out4[0] = o.i;
+  out4[1] = o.q;
+  out4[2] = o.u;
+  out4[3] = o.v;
+}
+
+// k6 = (alphaI, alphaQ, alphaV, rhoV, rhoQ, ds).
+void ref_stokes_step_full(const double* s4, const double* em4, const double* k6, double* out4) {
+  const physics::StokesVector s{s4[0], s4[1], s4[2], s4[3]};
+  const physics::StokesEmission em{em4[0], em4[1], em4[2], em4[3]};
+  const physics::FaradayPropagation k{k6[0], k6[1], k6[2], k6[3], k6[4], k6[5]};
+  const physics::StokesVector o = physics::stokesStepFull(s, em, k);
+  out4[0] = o.i;
+  out4[1] = o.q;
+  out4[2] = o.u;
+  out4[3] = o.v;
+}
+
+void ref_synch_pol_emission(double j_i, double pi_lin, double chi_b, double* out4) {
+  const physics::StokesEmission e = physics::synchrotronPolarizedEmission(j_i, pi_lin, chi_b);
+  out4[0] = e.jI;
+  out4[1] = e.jQ;
+  out4[2] = e.jU;
+  out4[3] = e.jV;
+}
+
+double ref_faraday_rotation_coeff(double nu, double ne, double b) {
+  return physics::faradayRotationCoeff(nu, ne, b);
+}
+
+double ref_synch_lin_pol_frac(double p) { return physics::synchrotronLinearPolarFrac(p); }
+
+double ref_thermal_lin_pol_frac(double theta_e) { return physics::thermalSynchLinearPolarFrac(theta_e); }
 
 }  // extern "C"

@@ -69,6 +69,51 @@ int main(int argc, char** argv) {
     for (const physics::RayTraceResult& r : rt.traceBatch(rays))
       std::printf("%d %d %.17g %.17g %.17g\n", static_cast<int>(r.status), r.stepsTaken, r.finalPosition[0],
                   r.finalPosition[1], r.finalPosition[2]);
+  } else if (!std::strcmp(what, "stokes")) {
+    // I Q U V jI jQ jU jV alphaI rhoV ds -> stokesStep
+    double v[11];
+    while (std::scanf("%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", v, v + 1, v + 2, v + 3, v + 4, v + 5, v + 6,
+                      v + 7, v + 8, v + 9, v + 10) == 11) {
+      const physics::StokesVector o =
+          physics::stokesStep({v[0], v[1], v[2], v[3]}, {v[4], v[5], v[6], v[7]}, v[8], v[9], v[10]);
+      std::printf("%.17g %.17g %.17g %.17g\n", o.i, o.q, o.u, o.v);
+    }
+  } else if (!std::strcmp(what, "stokes_full")) {
+    // I Q U V jI jQ jU jV alphaI alphaQ alphaV rhoV rhoQ ds -> stokesStepFull
+    double v[14];
+    while (std::scanf("%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", v, v + 1, v + 2, v + 3, v + 4,
+                      v + 5, v + 6, v + 7, v + 8, v + 9, v + 10, v + 11, v + 12, v + 13) == 14) {
+      const physics::StokesVector o = physics::stokesStepFull(
+          {v[0], v[1], v[2], v[3]}, {v[4], v[5], v[6], v[7]}, {v[8], v[9], v[10], v[11], v[12], v[13]});
+      std::printf("%.17g %.17g %.17g %.17g\n", o.i, o.q, o.u, o.v);
+    }
+  } else if (!std::strcmp(what, "stokes_misc")) {
+    // one line of scalar functions and vector helpers
+    const physics::StokesVector s{2.0, 0.6, 0.8, 0.3};
+    const physics::StokesVector r = s.rotateEvpa(0.3), f = physics::faradayRotateStokes(s, 0.3);
+    const physics::StokesVector g = physics::grTransformStokes(s, 0.8), z = physics::grTransformStokes(s, -1.0);
+    const physics::StokesEmission e = physics::synchrotronPolarizedEmission(2.0, 0.7, 0.4);
+    std::printf("%.17g %.17g %.17g %.17g %.17g\n", s.linPolInt(), s.linPolFrac(), s.circPolFrac(), s.totalPolFrac(),
+                s.evpa());
+    std::printf("%.17g %.17g %.17g %.17g\n", r.i, r.q, r.u, r.v);
+    std::printf("%.17g %.17g %.17g %.17g\n", f.i, f.q, f.u, f.v);
+    std::printf("%.17g %.17g %.17g %.17g\n", g.i, g.q, g.u, g.v);
+    std::printf("%.17g %.17g %.17g %.17g\n", z.i, z.q, z.u, z.v);
+    std::printf("%.17g %.17g %.17g %.17g\n", e.jI, e.jQ, e.jU, e.jV);
+    std::printf("%.17g %.17g %.17g %.17g\n", physics::synchrotronLinearPolarFrac(2.0),
+                physics::synchrotronLinearPolarFrac(3.0), physics::thermalSynchLinearPolarFrac(1.0),
+                physics::thermalSynchLinearPolarFrac(10.0));
+    std::printf("%d %d %d %.17g\n", physics::stokesPolarizationBound(s) ? 1 : 0,
+                physics::stokesPolarizationBound({1.0, 0.9, 0.5, 0.0}) ? 1 : 0,
+                physics::stokesPolarizationBound({1.0, 1.0, 0.0, 0.0}) ? 1 : 0,
+                physics::accumulatedRM({0.5, 1.5, -0.25}, {2.0, 1.0, 4.0}));
+  } else if (!std::strcmp(what, "faraday")) {
+    // nu ne B theta -> cold, relativistic, conversion
+    double nu, ne, b, th;
+    while (std::scanf("%lf %lf %lf %lf", &nu, &ne, &b, &th) == 4)
+      std::printf("%.17g %.17g %.17g\n", physics::faradayRotationCoeff(nu, ne, b),
+                  physics::faradayRotationCoeffRelativistic(nu, ne, b, th),
+                  physics::faradayConversionCoeff(nu, ne, b, th));
   } else if (!std::strcmp(what, "rayspec")) {
     double a, alpha, beta, th, r;
     while (std::scanf("%lf %lf %lf %lf %lf", &a, &alpha, &beta, &th, &r) == 5) {

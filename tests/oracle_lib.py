@@ -82,6 +82,14 @@ def lib():
         L.orc_trace_launch_params.restype = C.c_int
         L.orc_pinhole_setup.argtypes = [C.POINTER(abi.BH_LaunchParams), C.c_int64, C.c_int64, _PD, _PD]
         L.orc_pinhole_setup.restype = C.c_int
+        L.orc_trace_pol.argtypes = [
+            C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions),
+            _PD, C.c_int, C.POINTER(abi.BH_Volume), C.POINTER(C.c_float), _PD,
+            C.c_void_p, _PD, C.c_int,
+        ]
+        L.orc_trace_pol.restype = C.c_int
+        L.orc_pol_segment.argtypes = [_PD, _PD, _PD]
+        L.orc_pol_segment.restype = None
         _lib = L
     return _lib
 
@@ -124,6 +132,33 @@ def trace(cams, opts, luts=None, lut_n=0, vol=None, cells=None, ids=None,
     )
     assert rc == 0
     return out
+
+
+def trace_pol(cams, opts, luts, lut_n, vol, cells, evpa_b, lin_pol_frac, faraday_per_ne,
+              nthreads=0):
+    """Polarised oracle trace: (records, stokes[n, 4] = (I, Q, U, V))."""
+    cam_arr = (abi.BH_Camera * len(cams))(*cams)
+    n = max(c.record_offset + c.width * c.height for c in cams)
+    out = np.zeros(n, dtype=abi.RECORD_DTYPE)
+    st = np.zeros((n, 4))
+    pol = np.array([evpa_b, lin_pol_frac, faraday_per_ne], dtype=np.float64)
+    rc = lib().orc_trace_pol(
+        cam_arr, len(cams), C.byref(opts),
+        dptr(luts) if luts is not None else None, lut_n,
+        C.byref(vol), cells.ctypes.data_as(C.POINTER(C.c_float)), dptr(pol),
+        out.ctypes.data_as(C.c_void_p), dptr(st), nthreads)
+    assert rc == 0
+    return out, st
+
+
+def pol_segment(state, seg):
+    """One polarised step. state = (q, u, c, s, r0, Tr, a0, e0, s0);
+    seg = (h, a1, e1, r1, s1, cos 2chi, sin 2chi, frac); returns (q, u, c, s, r0)."""
+    si = np.ascontiguousarray(state, dtype=np.float64)
+    sg = np.ascontiguousarray(seg, dtype=np.float64)
+    so = np.zeros(5)
+    lib().orc_pol_segment(dptr(si), dptr(sg), dptr(so))
+    return so
 
 
 def trace_launch_params(params, nthreads=0) -> np.ndarray:

@@ -106,6 +106,34 @@ def test_null_arguments_rejected():
                                       C.c_void_p(1), None) == -1
 
 
+def test_polarised_entry_validation_without_gpu():
+    """bh_trace_image_plane_pol rejects its bad arguments on the host."""
+    import ctypes as C
+    from blackhole_amd import abi, tracer
+    from blackhole_amd._lib import lib
+    cams = (abi.BH_Camera * 1)(tracer.camera(0.5, 60.0, 100.0, 8, 8))
+    o = tracer.default_options(volume_enabled=1)
+    vol = abi.BH_Volume()
+    vol.cells = 1
+    pp = abi.BH_PolParams(0.3, 0.7, 1e-3, 0.0)
+    L = lib()
+    # NULL parameters, NULL Stokes buffer
+    assert L.bh_trace_image_plane_pol(cams, 1, C.byref(o), None, 0, C.byref(vol), None, C.c_void_p(1),
+                                      C.c_void_p(1), None) == -1
+    assert L.bh_trace_image_plane_pol(cams, 1, C.byref(o), None, 0, C.byref(vol), C.byref(pp), C.c_void_p(1),
+                                      None, None) == -1
+    # fraction outside [0, 1]
+    bad = abi.BH_PolParams(0.3, 1.5, 1e-3, 0.0)
+    assert L.bh_trace_image_plane_pol(cams, 1, C.byref(o), None, 0, C.byref(vol), C.byref(bad), C.c_void_p(1),
+                                      C.c_void_p(1), None) == -1
+    assert b"lin_pol_frac" in L.bh_last_error()
+    # no volume
+    o.volume_enabled = 0
+    assert L.bh_trace_image_plane_pol(cams, 1, C.byref(o), None, 0, None, C.byref(pp), C.c_void_p(1),
+                                      C.c_void_p(1), None) == -1
+    assert b"volume" in L.bh_last_error()
+
+
 def test_struct_layouts_match_the_c_headers(tmp_path):
     """sizeof / offsetof of every ABI struct from a C compile of include/*.h
     against the ctypes mirrors."""
@@ -119,6 +147,7 @@ def test_struct_layouts_match_the_c_headers(tmp_path):
         "BH_VolumeSet": abi.BH_VolumeSet, "BH_TuneResult": abi.BH_TuneResult, "BH_RaySpec": abi.BH_RaySpec,
         "BH_SurrogateRenderParams": S.RenderParams, "BH_SurrogateConfig": S.SurrogateConfig,
         "BH_TorusParams": abi.BH_TorusParams, "BH_GrmhdPacked": G.BH_GrmhdPacked,
+        "BH_PolParams": abi.BH_PolParams,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bhk_trace.h"', '#include "bhk_surrogate.h"',
              '#include "bh_grmhd.h"',

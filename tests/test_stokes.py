@@ -1,0 +1,363 @@
+"""Polarised transfer on the CPU: the physics interface's Stokes functions
+(include/bh_physics.hpp) and the oracle's polarised accumulation.
+
+Pinned by
+  * the known answers of the reference's tests/stokes_transport_test.cpp
+    (Faraday rotation angle, EVPA rotation by rho_V ds / 2, pure absorption,
+    thick-slab limit, scalar-RTE agreement, power-law fractions 9/13 and 3/4,
+    E vector perpendicular to B, coefficient scalings),
+  * the reference's own stokesStep / stokesStepFull / emission functions
+    compiled from src/physics/stokes_transport.h into oracle/_ref/libref.so
+    (skipped where absent),
+  * the forward composition of exact segment steps (the reference's order:
+    far end first) against the oracle's backward accumulation (observer
+    first, transmittance and Faraday angle carried), which a wrong rotation
+    sign, a dropped transmittance or a swapped Q/U fails,
+  * scipy's modified Bessel functions for the thermal Faraday factors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+_PD = C.POINTER(C.c_double)
+_D = C.c_double
+
+
+def P(a):
+    return a.ctypes.data_as(_PD)
+
+
+@pytest.fixture(scope="module")
+def prog(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("stokes") / "physics_api_check"
+    lib_dir = os.path.join(ROOT, "blackhole_amd")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "physics_api_check.cpp"), "-o", str(exe),
+                    "-L", lib_dir, "-lbhk", f"-Wl,-rpath,{lib_dir}"], check=True)
+
+    def run(what, rows=None):
+        stdin = "" if rows is None else "\n".join(" ".join(f"{v:.17g}" for v in r) for r in rows)
+        out = subprocess.run([str(exe), what], input=stdin, capture_output=True, text=True, check=True).stdout
+        rows = [np.array([float(v) for v in l.split()]) for l in out.strip().split("\n")]
+        return np.array(rows) if len({len(r) for r in rows}) == 1 else rows
+    return run
+
+
+@pytest.fixture(scope="module")
+def ref():
+    if not os.path.exists(REF_LIB):
+        pytest.skip("oracle/_ref/libref.so not built (needs /root/reference)")
+    R = C.CDLL(REF_LIB)
+    R.ref_stokes_step.argtypes = [_PD, _PD, _D, _D, _D, _PD]
+    R.ref_stokes_step_full.argtypes = [_PD, _PD, _PD, _PD]
+    R.ref_synch_pol_emission.argtypes = [_D, _D, _D, _PD]
+    R.ref_faraday_rotation_coeff.argtypes = [_D, _D, _D]
+    R.ref_faraday_rotation_coeff.restype = _D
+    R.ref_synch_lin_pol_frac.argtypes = [_D]
+    R.ref_synch_lin_pol_frac.restype = _D
+    R.ref_thermal_lin_pol_frac.argtypes = [_D]
+    R.ref_thermal_lin_pol_frac.restype = _D
+    return R
+
+
+def step(prog, s, em, a, r, ds):
+    return prog("stokes", [list(s) + list(em) + [a, r, ds]])[0]
+
+
+# ---------------------------------------------------------------------------
+# known answers of the reference's stokes_transport_test.cpp
+# ---------------------------------------------------------------------------
+
+def test_faraday_rotation_angle_and_evpa(prog):
+    rho, ds = 0.7, 1.3
+    phi = rho * ds
+    o = step(prog, (1.0, 1.0, 0.0, 0.0), (0, 0, 0, 0), 0.0, rho, ds)
+    assert abs(o[1] - math.cos(phi)) <= 1e-12 and abs(o[2] - math.sin(phi)) <= 1e-12
+    assert o[0] == 1.0 and o[3] == 0.0
+    # EVPA rotates by rho ds / 2; linear fraction conserved
+    s0 = (2.0, 0.6, -0.3, 0.1)
+    o = step(prog, s0, (0, 0, 0, 0), 0.0, rho, ds)
+    chi0 = 0.5 * math.atan2(s0[2], s0[1])
+    chi1 = 0.5 * math.atan2(o[2], o[1])
+    assert abs(chi1 - chi0 - 0.5 * phi) <= 1e-12
+    assert abs(math.hypot(o[1], o[2]) - math.hypot(s0[1], s0[2])) <= 1e-14
+    assert o[0] == s0[0] and o[3] == s0[3]
+
+
+def test_pure_absorption_and_thick_slab(prog):
+    s0 = (1.0, 0.4, -0.3, 0.2)
+    a, ds = 0.8, 2.5
+    o = step(prog, s0, (0, 0, 0, 0), a, 0.0, ds)
+    d = math.exp(-a * ds)
+    np.testing.assert_allclose(o, np.array(s0) * d, rtol=1e-14)
+    # optically thick: I -> j / alpha, (Q, U) -> (jQ, jU) / alpha without rotation
+    em = (3.0, -1.2, 0.9, 0.0)
+    o = step(prog, (0, 0, 0, 0), em, 2.0, 0.0, 50.0)
+    np.testing.assert_allclose(o[:3], np.array(em[:3]) / 2.0, rtol=1e-14)
+    # thick with rotation: P -> j_P / (A - i R)
+    A, R = 2.0, 1.5
+    o = step(prog, (0, 0, 0, 0), em, A, R, 60.0)
+    want = complex(em[1], em[2]) / complex(A, -R)
+    assert abs(complex(o[1], o[2]) - want) <= 1e-14 * abs(want)
+
+
+def test_scalar_channel_matches_rte_step(prog):
+    """I obeys the scalar RTE (the oracle's rte_step, pinned by the
+    reference's rte_integrator tests) at every optical depth."""
+    L = OL.lib()
+    for a, ds in ((0.0, 1.0), (1e-7, 2.0), (0.3, 0.5), (5.0, 3.0)):
+        st = np.array([0.7, 0.0, 0.0])
+        L.orc_rte_step(OL.dptr(st), 2.0, a, ds)
+        o = step(prog, (0.7, 0, 0, 0), (2.0, 0, 0, 0), a, 0.0, ds)
+        assert abs(o[0] - st[0]) <= 1e-13 * st[0], (a, ds)
+        assert o[1] == 0.0 and o[2] == 0.0
+
+
+def test_zero_step_and_positivity(prog):
+    s0 = (1.0, 0.2, 0.3, 0.1)
+    for ds in (0.0, -1.0):
+        np.testing.assert_array_equal(step(prog, s0, (1, 1, 1, 1), 0.5, 0.5, ds), s0)
+    assert step(prog, (0, 0, 0, 0), (1e-3, 0, 0, 0), 0.4, 0.2, 1.0)[0] > 0.0
+
+
+def test_exact_step_converges_to_rk4(prog):
+    """Fine RK4 substeps of the simplified K converge to the exact step
+    (fourth order)."""
+    s0, em, A, R, ds = (1.0, 0.3, -0.2, 0.05), (0.5, -0.2, 0.15, 0.01), 0.6, 1.7, 1.0
+    exact = step(prog, s0, em, A, R, ds)
+    errs = []
+    for n in (16, 32):
+        s = list(s0)
+        for _ in range(n):
+            s = list(prog("stokes_full", [s + list(em) + [A, 0, 0, R, 0, ds / n]])[0])
+        errs.append(np.abs(np.array(s) - exact).max())
+    assert errs[1] < 1e-7 and 13.0 < errs[0] / errs[1] < 19.0, errs
+
+
+def test_scalar_helpers_and_emission(prog):
+    m = prog("stokes_misc")
+    s = (2.0, 0.6, 0.8, 0.3)
+    assert abs(m[0][0] - 1.0) <= 1e-15 and abs(m[0][1] - 0.5) <= 1e-15
+    assert abs(m[0][2] - 0.15) <= 1e-15 and abs(m[0][3] - math.sqrt(1.09) / 2) <= 1e-15
+    assert abs(m[0][4] - 0.5 * math.atan2(0.8, 0.6)) <= 1e-15
+    c, sn = math.cos(0.6), math.sin(0.6)
+    rot = (2.0, 0.6 * c - 0.8 * sn, 0.6 * sn + 0.8 * c, 0.3)
+    np.testing.assert_allclose(m[1], rot, atol=1e-15)
+    np.testing.assert_allclose(m[2], rot, atol=1e-15)          # faradayRotateStokes
+    np.testing.assert_allclose(m[3], np.array(s) * 0.512, rtol=1e-15)
+    np.testing.assert_array_equal(m[4], 0.0)
+    # emission: fraction 0.7, E vector perpendicular to B (EVPA = chi_B + pi/2)
+    jI, jQ, jU, jV = m[5]
+    assert jI == 2.0 and jV == 0.0 and abs(math.hypot(jQ, jU) / jI - 0.7) <= 1e-15
+    evpa = 0.5 * math.atan2(jU, jQ)
+    assert abs(math.remainder(evpa - (0.4 + 0.5 * math.pi), math.pi)) <= 1e-14
+    # power-law fractions 9/13 and 3/4; thermal fraction in (0.6, 1)
+    assert abs(m[6][0] - 9.0 / 13.0) <= 1e-15 and abs(m[6][1] - 0.75) <= 1e-15
+    assert 0.6 < m[6][2] < 1.0 and 0.6 < m[6][3] < 1.0 and m[6][3] > m[6][2]
+    assert list(m[7][:3]) == [1.0, 0.0, 1.0]
+    assert m[7][3] == 0.5 * 2.0 + 1.5 * 1.0 - 0.25 * 4.0
+
+
+def test_faraday_coefficients(prog):
+    sps = pytest.importorskip("scipy.special")
+    rows = [(230e9, 1e6, 10.0, 0.5), (230e9, 2e6, 10.0, 3.0), (460e9, 1e6, 20.0, 10.0), (86e9, 1e5, 5.0, 1.0),
+            (0.0, 1e6, 1.0, 1.0), (230e9, 0.0, 1.0, 1.0)]
+    out = prog("faraday", rows)
+    e, me, c = 4.80320425e-10, 9.1093837015e-28, 2.99792458e10
+    for (nu, ne, b, th), (cold, rel, conv) in zip(rows, out):
+        if nu <= 0 or ne <= 0:
+            assert cold == rel == conv == 0.0
+            continue
+        want = e ** 3 / (2 * math.pi * me * me * c ** 4) * ne * b / nu ** 2
+        assert abs(cold - want) <= 1e-13 * want
+        z = 1.0 / th
+        k0, k1, k2 = (sps.kve(n, z) for n in (0, 1, 2))
+        assert abs(rel - want * (k0 + k1) / (2 * th * th * k2)) <= 1e-11 * abs(rel)
+        fc = max(0.0, k1 / (th * k2) - 0.5 / th ** 2)
+        assert abs(conv - 2 * want * fc) <= 1e-11 * max(abs(conv), 1e-300)
+    # the reference's scalings: linear in n_e and B, 1 / nu^2; suppressed when hot
+    assert abs(out[1, 0] / out[0, 0] - 2.0) <= 1e-13
+    assert out[1, 1] < out[1, 0] and out[1, 2] >= 0.0
+
+
+def test_stokes_step_matches_reference(prog, ref):
+    """Random states in every regime against the reference's stokesStep.
+    Where the reference takes its first-order step (alpha ds < 1e-4: I and V
+    become S (1 - alpha ds) + j ds, without the emission's -j ds alpha ds / 2)
+    the bound is that step's truncation, alpha ds (|S| + |j| ds)."""
+    rng = np.random.default_rng(11)
+    rows = []
+    for k in range(400):
+        s = list(rng.normal(size=4))
+        s[0] = abs(s[0]) + 1.0
+        em = list(rng.normal(size=4))
+        em[0] = abs(em[0])
+        regime = k % 5
+        A = [rng.uniform(0.01, 3.0), 0.0, rng.uniform(0.01, 3.0), 10 ** rng.uniform(-9, -5), 0.0][regime]
+        R = [rng.normal() * 2, rng.normal() * 2, 0.0, rng.normal(), 0.0][regime]
+        ds = rng.uniform(0.01, 3.0)
+        rows.append(s + em + [A, R, ds])
+    got = prog("stokes", rows)
+    for row, g in zip(rows, got):
+        s, em, (A, R, ds) = np.array(row[:4]), np.array(row[4:8]), row[8:]
+        want = np.zeros(4)
+        ref.ref_stokes_step(P(s), P(em), A, R, ds, P(want))
+        scale = np.abs(s).sum() + np.abs(em).sum() * ds
+        tol = 1e-13 * scale + (A * ds if A * ds < 1e-4 else 0.0) * scale
+        assert np.abs(g - want).max() <= tol, (row, g, want)
+
+
+def test_rk4_and_emission_match_reference(prog, ref):
+    rng = np.random.default_rng(12)
+    for _ in range(50):
+        s = rng.normal(size=4)
+        em = rng.normal(size=4)
+        k = np.array([rng.uniform(0, 2), rng.normal() * 0.3, rng.normal() * 0.3, rng.normal(), rng.normal(),
+                      rng.uniform(0, 0.5)])
+        got = prog("stokes_full", [list(s) + list(em) + list(k)])[0]
+        want = np.zeros(4)
+        ref.ref_stokes_step_full(P(s), P(em), P(k), P(want))
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-14)
+    m = prog("stokes_misc")
+    want = np.zeros(4)
+    ref.ref_synch_pol_emission(2.0, 0.7, 0.4, P(want))
+    np.testing.assert_allclose(m[5], want, rtol=1e-15)
+    assert abs(m[6][0] - ref.ref_synch_lin_pol_frac(2.0)) <= 1e-15
+    assert abs(m[6][2] - ref.ref_thermal_lin_pol_frac(1.0)) <= 1e-15
+    assert abs(m[6][3] - ref.ref_thermal_lin_pol_frac(10.0)) <= 1e-15
+    out = prog("faraday", [(230e9, 1e6, 10.0, 1.0)])
+    assert abs(out[0, 0] - ref.ref_faraday_rotation_coeff(230e9, 1e6, 10.0)) <= 1e-14 * out[0, 0]
+
+
+# ---------------------------------------------------------------------------
+# the oracle's backward accumulation against forward composition
+# ---------------------------------------------------------------------------
+
+def _path(rng, n, rot_scale=1.0):
+    A = rng.uniform(0.0, 0.8, n)
+    A[rng.uniform(size=n) < 0.2] = 0.0
+    R = rng.normal(size=n) * rot_scale
+    J = rng.uniform(0.0, 2.0, n)
+    J[rng.uniform(size=n) < 0.2] = 0.0
+    L = rng.uniform(0.05, 1.5, n)
+    return A, R, J, L
+
+
+def _backward(A, R, J, L, chi, frac):
+    """The oracle's pol_segment / rt_segment chain, observer first."""
+    c2, s2 = math.cos(2 * chi), math.sin(2 * chi)
+    q = u = 0.0
+    c, s = 1.0, 0.0
+    tr = np.zeros(6)
+    tr[1] = 1.0
+    for k in range(len(A)):
+        st = np.array([q, u, c, s, R[k], tr[1], A[k], J[k], 1.0])
+        q, u, c, s, _ = OL.pol_segment(st, (L[k], A[k], J[k], R[k], 1.0, c2, s2, frac))
+        tr[3], tr[4], tr[5] = A[k], J[k], 1.0
+        OL.lib().orc_rt_segment(OL.dptr(tr), L[k], A[k], J[k], 1.0)
+    return tr[0], q, u
+
+
+def test_oracle_pol_accumulation_equals_forward_composition(prog):
+    rng = np.random.default_rng(5)
+    for trial in range(40):
+        n = int(rng.integers(1, 30))
+        A, R, J, L = _path(rng, n, rot_scale=[0.0, 0.01, 1.0, 20.0][trial % 4])
+        chi, frac = rng.uniform(-1.5, 1.5), rng.uniform(0.0, 1.0)
+        I_b, q_b, u_b = _backward(A, R, J, L, chi, frac)
+        rows, s = [], [0.0, 0.0, 0.0, 0.0]
+        # forward: the segment farthest from the observer first
+        for k in reversed(range(n)):
+            em = [J[k], -frac * J[k] * math.cos(2 * chi), -frac * J[k] * math.sin(2 * chi), 0.0]
+            s = list(prog("stokes", [s + em + [A[k], R[k], L[k]]])[0])
+        scale = max(J.max() * L.sum(), 1e-300)
+        assert abs(I_b - s[0]) <= 1e-13 * scale, trial
+        assert abs(q_b - s[1]) <= 1e-12 * scale and abs(u_b - s[2]) <= 1e-12 * scale, (trial, q_b, u_b, s)
+
+
+def test_oracle_pol_accumulation_matches_reference_forward(ref):
+    rng = np.random.default_rng(6)
+    for trial in range(20):
+        n = int(rng.integers(2, 25))
+        A, R, J, L = _path(rng, n)
+        A = np.maximum(A, 0.01)       # outside the reference's first-order branch
+        chi, frac = rng.uniform(-1.5, 1.5), rng.uniform(0.2, 1.0)
+        I_b, q_b, u_b = _backward(A, R, J, L, chi, frac)
+        s = np.zeros(4)
+        for k in reversed(range(n)):
+            em = np.zeros(4)
+            ref.ref_synch_pol_emission(J[k], frac, chi - 0.5 * math.pi, P(em))
+            o = np.zeros(4)
+            ref.ref_stokes_step(P(s), P(em), A[k], R[k], L[k], P(o))
+            s = o
+        scale = J.max() * L.sum()
+        # the reference's emission for chi_B = chi - pi/2 has j_Q + i j_U = -frac j e^{2 i chi_B}
+        # = +frac j e^{2 i chi}: the opposite sign of the tracer's source for EVPA chi
+        assert abs(I_b - s[0]) <= 1e-12 * scale
+        assert abs(q_b + s[1]) <= 1e-12 * scale and abs(u_b + s[2]) <= 1e-12 * scale, trial
+
+
+def test_pol_segment_series_branch_is_continuous():
+    """Across |w|^2 = 1e-6 (series below, closed form above) the step differs
+    by less than the series' truncation."""
+    out = []
+    for dl in (0.999e-3, 1.001e-3):
+        st = np.array([0.0, 0.0, 1.0, 0.0, 0.6, 1.0, 0.8, 1.0, 1.0])
+        out.append(OL.pol_segment(st, (dl, 0.8, 1.0, 0.6, 1.0, 1.0, 0.0, 1.0)) / dl)
+    assert np.abs(out[0][:2] - out[1][:2]).max() <= 5e-6   # the step itself varies by O(dl)
+
+
+# ---------------------------------------------------------------------------
+# whole-ray oracle invariants
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def scene():
+    from blackhole_amd import workloads
+    return workloads.c2(24, 24, grid=(64, 32, 16))
+
+
+def _trace_pol(w, chi, frac, far):
+    return OL.trace_pol(w["cams"], w["opts"], None, 0, w["vol"], w["cells"], chi, frac, far)
+
+
+def test_oracle_trace_pol_invariants(scene):
+    w = scene
+    base = OL.trace(w["cams"], w["opts"], vol=w["vol"], cells=w["cells"])
+    rec0, st0 = _trace_pol(w, 0.3, 0.7, 0.0)
+    # the records are the unpolarised trace's, I is the record's intensity
+    assert np.array_equal(rec0.view(np.uint8), base.view(np.uint8))
+    assert np.array_equal(st0[:, 0], base["intensity"]) and np.all(st0[:, 3] == 0.0)
+    I = st0[:, 0]
+    lit = I > 1e-6 * I.max()
+    assert lit.sum() > 50
+    # no Faraday rotation: Q + iU = -frac I e^{2 i chi} (same weights as I)
+    want = -0.7 * I * np.exp(2j * 0.3)
+    got = st0[:, 1] + 1j * st0[:, 2]
+    assert np.abs(got - want).max() <= 1e-10 * I.max()
+    # the source phase: P(chi) = e^{2 i (chi - chi0)} P(chi0) at any Faraday depth
+    far = 1e-12
+    _, sa = _trace_pol(w, 0.3, 0.7, far)
+    _, sb = _trace_pol(w, -0.5, 0.7, far)
+    pa, pb = sa[:, 1] + 1j * sa[:, 2], sb[:, 1] + 1j * sb[:, 2]
+    assert np.abs(pb - pa * np.exp(2j * (-0.8))).max() <= 1e-10 * I.max()
+    # depolarisation: |P| <= frac I, strictly below where the rotation acts
+    m = np.abs(pa)
+    assert np.all(m <= 0.7 * I * (1 + 1e-10))   # series vs expm1 weights: ~2e-11
+    frac_mean = [float((np.abs(_trace_pol(w, 0.3, 0.7, f)[1][:, 1:3] @ [1, 1j])[lit] / I[lit]).mean())
+                 for f in (0.0, far, 10 * far)]
+    assert abs(frac_mean[0] - 0.7) <= 1e-9 and frac_mean[2] < 0.2
+    assert frac_mean[0] > frac_mean[1] > frac_mean[2], frac_mean
+    # linear in the fraction
+    _, sh = _trace_pol(w, 0.3, 0.35, far)
+    assert np.abs(sh[:, 1:3] - 0.5 * sa[:, 1:3]).max() <= 1e-12 * I.max()
