@@ -165,41 +165,53 @@ __device__ __forceinline__ void store_act(const WaveAcc& acc, uint16_t* __restri
     }
 }
 
+// Weight fragments of one K-step for the wave's two feature blocks.  Wsw is
+// a layer in fragment order: for feature block fb = o0 / 32 + ob and K-step
+// ks, lane l's 8 weights sit at Wsw[((fb * 16 + ks) * 64 + l) * 8].
+struct AFrag {
+  bf16x8 a0, a1;
+};
+__device__ __forceinline__ AFrag load_afrag(const uint16_t* __restrict__ Wsw, int o0, int lane, int ks) {
+  const uint16_t* wp0 = Wsw + ((o0 / 32) * 16 * 64 + lane) * 8 + ks * 512;
+  return AFrag{ld16(wp0), ld16(wp0 + 16 * 64 * 8)};
+}
+
+// Weight fragments stream from L2 kPrefetch K-steps ahead of their MFMAs
+// (a ring of kPrefetch AFrags; kPrefetch divides the 16 K-steps), and each
+// K-step's activation fragments are read from LDS one step ahead.  The
+// compiler's scheduler would otherwise sink every load next to its first use
+// and expose the L2 and LDS latencies on every K-step; a scheduling barrier
+// after each step's loads keeps them where they are issued.
+constexpr int kPrefetch = 4;
+static_assert((kW / 16) % kPrefetch == 0, "the prefetch ring must divide the K-steps");
+
 // One hidden layer's accumulation: acc = bias + W[o0 + ..][:] . H[ray][:]^T,
-// 16 K-steps of 16.  Wsw is the layer in fragment order: for feature block
-// fb = o0 / 32 + ob and K-step ks, lane l's 8 weights sit at
-// Wsw[((fb * 16 + ks) * 64 + l) * 8].  The first K-step reads the bias as the
-// MFMA's C operand; the next K-step's weight fragments load one step ahead.
+// 16 K-steps of 16, with the first kPrefetch K-steps' weights already in pa;
+// the ring is refilled kPrefetch steps ahead and left empty.  The first
+// K-step reads the bias as the MFMA's C operand.
 __device__ __forceinline__ void hidden_mfma(WaveAcc& acc, const uint16_t* __restrict__ Wsw,
-                                            const uint16_t* __restrict__ H, const f32x16 (&b16)[2], int o0,
-                                            int lane, int l32, int lh) {
-  const uint16_t* wp0 = Wsw + ((o0 / 32) * 16 * 64 + lane) * 8;
-  const uint16_t* wp1 = wp0 + 16 * 64 * 8;
+                                            const uint16_t* __restrict__ H, const f32x16 (&b16)[2],
+                                            AFrag (&pa)[kPrefetch], int o0, int lane, int l32, int lh) {
   const uint16_t* hp = H + l32 * kLdH + 8 * lh;
-  bf16x8 an0 = ld16(wp0), an1 = ld16(wp1);
-  {
-    const bf16x8 a0 = an0, a1 = an1;
-    an0 = ld16(wp0 + 512);
-    an1 = ld16(wp1 + 512);
+  bf16x8 bn[4];
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb) {
-      const bf16x8 bv = ld16(hp + 32 * rb * kLdH);
-      acc[0][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bv, b16[0], 0, 0, 0);
-      acc[1][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bv, b16[1], 0, 0, 0);
-    }
-  }
-#pragma unroll 3
-  for (int ks = 1; ks < kW / 16; ++ks) {
-    const bf16x8 a0 = an0, a1 = an1;
+  for (int rb = 0; rb < 4; ++rb) bn[rb] = ld16(hp + 32 * rb * kLdH);
+#pragma unroll
+  for (int ks = 0; ks < kW / 16; ++ks) {
+    const AFrag a = pa[ks % kPrefetch];
+    if (ks + kPrefetch < kW / 16) pa[ks % kPrefetch] = load_afrag(Wsw, o0, lane, ks + kPrefetch);
+    bf16x8 bv[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) bv[rb] = bn[rb];
     if (ks + 1 < kW / 16) {
-      an0 = ld16(wp0 + (ks + 1) * 512);
-      an1 = ld16(wp1 + (ks + 1) * 512);
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) bn[rb] = ld16(hp + 32 * rb * kLdH + 16 * (ks + 1));
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
-      const bf16x8 bv = ld16(hp + 32 * rb * kLdH + 16 * ks);
-      acc[0][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bv, acc[0][rb], 0, 0, 0);
-      acc[1][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bv, acc[1][rb], 0, 0, 0);
+      acc[0][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.a0, bv[rb], ks == 0 ? b16[0] : acc[0][rb], 0, 0, 0);
+      acc[1][rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.a1, bv[rb], ks == 0 ? b16[1] : acc[1][rb], 0, 0, 0);
     }
   }
 }
@@ -276,13 +288,21 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
       }
     }
   }
+  AFrag pa[kPrefetch];  // the next layer's first K-steps, in flight across the epilogue
+#pragma unroll
+  for (int d = 0; d < kPrefetch; ++d) pa[d] = load_afrag(P.wsw, o0, lane, d);
   store_act(acc, H, o0, l32, lh);
 
   // ---- hidden layers 1..n_hidden: 256 x 256 ----------------------------------------
   for (int l = 1; l <= P.n_hidden; ++l) {
+    const uint16_t* Wl = P.wsw + static_cast<size_t>(l - 1) * kW * kW;
     load_bias(b16, P.bs + l * kW, o0, lh);
     __syncthreads();  // H complete
-    hidden_mfma(acc, P.wsw + static_cast<size_t>(l - 1) * kW * kW, H, b16, o0, lane, l32, lh);
+    hidden_mfma(acc, Wl, H, b16, pa, o0, lane, l32, lh);
+    if (l < P.n_hidden) {
+#pragma unroll
+      for (int d = 0; d < kPrefetch; ++d) pa[d] = load_afrag(Wl + kW * kW, o0, lane, d);
+    }
     __syncthreads();  // every wave has read H
     store_act(acc, H, o0, l32, lh);
   }
