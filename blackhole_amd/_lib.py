@@ -96,6 +96,8 @@ def lib():
     L.bh_trace_disk_mino.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.BH_TraceOptions), C.c_void_p, C.c_int,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
     L.bh_trace_disk_mino.restype = C.c_int
+    L.bh_trace_disk_mino_refine.argtypes = L.bh_trace_disk_mino.argtypes
+    L.bh_trace_disk_mino_refine.restype = C.c_int
     L.bh_trace_image_plane_pol.argtypes = [
         C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions), C.c_void_p, C.c_int,
         C.POINTER(abi.BH_Volume), C.POINTER(abi.BH_PolParams), C.c_void_p, C.c_void_p, C.c_void_p]

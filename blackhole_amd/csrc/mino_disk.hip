@@ -129,23 +129,26 @@ std::map<std::pair<int, void*>, CamBuf> g_bufs;
 
 using namespace bhk;
 
-extern "C" {
+namespace {
 
-int bh_trace_disk_mino(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts, const double* disk_luts,
-                       int disk_lut_n, BH_PixelRecord* out, int64_t* fallback_ids, uint64_t* fallback_count,
-                       int64_t fallback_cap, void* stream) {
+// Both entry points: refine = 0 solves every pixel and starts a new fallback
+// list; refine = 1 solves the surrogate's records in place from their radii
+// and appends its failures to the list the render started.
+int mino_entry(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts, const double* disk_luts, int disk_lut_n,
+               BH_PixelRecord* out, int64_t* fallback_ids, uint64_t* fallback_count, int64_t fallback_cap,
+               void* stream, int refine, const char* who) {
+  const std::string w(who);
   if (!cams || n_cams <= 0 || !opts || !disk_luts || disk_lut_n < 2 || !out || !fallback_ids || !fallback_count)
-    return fail("bh_trace_disk_mino: NULL argument");
-  if (fallback_cap <= 0) return fail("bh_trace_disk_mino: fallback_cap must be > 0");
+    return fail(w + ": NULL argument");
+  if (fallback_cap <= 0) return fail(w + ": fallback_cap must be > 0");
   if (!opts->disk_enabled || opts->volume_enabled)
-    return fail("bh_trace_disk_mino: thin disk only (disk_enabled = 1, volume_enabled = 0)");
-  if (!(opts->disk_r_out > 0.0) || !(opts->r_escape > 0.0))
-    return fail("bh_trace_disk_mino: disk_r_out and r_escape must be > 0");
+    return fail(w + ": thin disk only (disk_enabled = 1, volume_enabled = 0)");
+  if (!(opts->disk_r_out > 0.0) || !(opts->r_escape > 0.0)) return fail(w + ": disk_r_out and r_escape must be > 0");
   std::vector<CamDev> hc;
   long long total = 0;
-  if (make_cam_devs(cams, n_cams, opts->horizon_factor, "bh_trace_disk_mino", &hc, &total) != 0) return -1;
+  if (make_cam_devs(cams, n_cams, opts->horizon_factor, who, &hc, &total) != 0) return -1;
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return fail("bh_trace_disk_mino: no HIP device");
+  if (hipGetDevice(&dev) != hipSuccess) return fail(w + ": no HIP device");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const size_t bytes = hc.size() * sizeof(CamDev);
   CamDev* d_cams;
@@ -157,14 +160,14 @@ int bh_trace_disk_mino(const BH_Camera* cams, int n_cams, const BH_TraceOptions*
       b.d = nullptr;
       b.cap = 0;
       const size_t cap = std::max<size_t>(bytes, 64 * sizeof(CamDev));
-      if (hipMalloc(&b.d, cap) != hipSuccess) return fail("bh_trace_disk_mino: hipMalloc failed");
+      if (hipMalloc(&b.d, cap) != hipSuccess) return fail(w + ": hipMalloc failed");
       b.cap = cap;
     }
     d_cams = b.d;
   }
   // Pageable source: the runtime stages it before returning.
   if (hipMemcpyAsync(d_cams, hc.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess)
-    return fail("bh_trace_disk_mino: camera upload failed");
+    return fail(w + ": camera upload failed");
   MinoParams P;
   std::memset(&P, 0, sizeof(P));
   P.cams = d_cams;
@@ -181,11 +184,29 @@ int bh_trace_disk_mino(const BH_Camera* cams, int n_cams, const BH_TraceOptions*
   P.tol = kMinoTol;
   P.n_cams = n_cams;
   P.lut_n = disk_lut_n;
-  P.refine = 0;
-  hipLaunchKernelGGL(k_mino_zero, dim3(1), dim3(64), 0, st, P.fb_count);
-  if (total / 256 + 1 > 0x7fffffffll) return fail("bh_trace_disk_mino: too many pixels");
+  P.refine = refine;
+  if (total / 256 + 1 > 0x7fffffffll) return fail(w + ": too many pixels");
+  if (!refine) hipLaunchKernelGGL(k_mino_zero, dim3(1), dim3(64), 0, st, P.fb_count);
   const hipError_t e = launch_mino_disk(P, st);
-  return e == hipSuccess ? 0 : fail(std::string("bh_trace_disk_mino: launch failed: ") + hipGetErrorString(e));
+  return e == hipSuccess ? 0 : fail(w + ": launch failed: " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+extern "C" {
+
+int bh_trace_disk_mino(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts, const double* disk_luts,
+                       int disk_lut_n, BH_PixelRecord* out, int64_t* fallback_ids, uint64_t* fallback_count,
+                       int64_t fallback_cap, void* stream) {
+  return mino_entry(cams, n_cams, opts, disk_luts, disk_lut_n, out, fallback_ids, fallback_count, fallback_cap,
+                    stream, 0, "bh_trace_disk_mino");
+}
+
+int bh_trace_disk_mino_refine(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+                              const double* disk_luts, int disk_lut_n, BH_PixelRecord* out, int64_t* fallback_ids,
+                              uint64_t* fallback_count, int64_t fallback_cap, void* stream) {
+  return mino_entry(cams, n_cams, opts, disk_luts, disk_lut_n, out, fallback_ids, fallback_count, fallback_cap,
+                    stream, 1, "bh_trace_disk_mino_refine");
 }
 
 }  // extern "C"

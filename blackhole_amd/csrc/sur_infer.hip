@@ -637,7 +637,8 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
       !fallback_ids || !fallback_count || !audit_ids || !audit_count)
     return fail("bh_surrogate_render: NULL argument");
   if (fallback_cap <= 0 || audit_cap <= 0) return fail("bh_surrogate_render: list capacities must be > 0");
-  if (!(rp->ring_band >= 0.0f && rp->edge_tol >= 0.0f && rp->ambiguity >= 0.0f) || rp->audit_every < 0)
+  if (!(rp->ring_band >= 0.0f && rp->edge_tol >= 0.0f && rp->ambiguity >= 0.0f) || rp->audit_every < 0 ||
+      rp->refine < 0 || rp->refine > 2)
     return fail("bh_surrogate_render: invalid render parameters");
   const SurView v = surrogate_view(s);
   if (v.width != kW || v.L < 2 || v.L > kMaxLayers)
@@ -719,7 +720,7 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
   if (nblk > 0x7fffffffll) return fail("bh_surrogate_render: too many pixels");
   hipLaunchKernelGGL(k_sur_render, dim3(static_cast<unsigned>(nblk)), dim3(256), 0, st, P);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess && P.refine) {
+  if (e == hipSuccess && rp->refine == 1) {
     MinoParams M;
     std::memset(&M, 0, sizeof(M));
     M.cams = d_cams;

@@ -11,7 +11,11 @@ One frame set (any number of cameras at r_obs = 100 M) goes through:
      into the same image (records overwrite the surrogate's);
   3. bh_trace_pixel_list_compact + bh_surrogate_audit: the audit sample
      traced into a compact buffer and compared with the surrogate records.
-All steps are asynchronous on one stream; list lengths stay on the device.
+All steps are asynchronous; list lengths stay on the device.  With
+overlap_audit, the network pass leaves the solve to bh_trace_disk_mino_refine
+so that the audit sample (which needs only the network's audit list) is
+traced on a second stream beside the solve and the fallback, and joined
+back before the comparison.
 """
 from __future__ import annotations
 
@@ -35,6 +39,9 @@ class RenderConfig:
     ambiguity: float = 0.3
     audit_every: int = 100      # 1 % audit
     refine: bool = True         # exact Mino-time crossing solve after the network
+    # trace the audit sample on a second stream beside the solve; off by default:
+    # on c5 the concurrent FP64 kernels contend (63.5-64.1 ms against 61.6-62.5 ms serial)
+    overlap_audit: bool = False
     intensity_floor_rel: float = 1e-6  # audit compares pixels with I_ref > floor x max I
 
 
@@ -68,15 +75,30 @@ class SurrogateRenderer:
         self.audit_rec = tracer.alloc_records(self.audit_cap, device=dev)
         self.stats = torch.zeros(8, dtype=torch.float64, device=dev)
         self.ring = torch.from_numpy(ring_tables(scene.cams).reshape(-1)).to(dev)
+        split = cfg.refine and cfg.overlap_audit and cfg.audit_every > 0
+        self.side = torch.cuda.Stream(device=dev) if split else None  # default priority
         self.params = S.RenderParams(cfg.ring_band, cfg.edge_tol, cfg.conf_max, cfg.ambiguity,
-                                     cfg.audit_every, int(cfg.refine))
+                                     cfg.audit_every, 2 if split else int(cfg.refine))
         self.floor_I = 0.0
 
     def _stream(self):
         return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
 
     def render(self) -> None:
-        """Surrogate pass only (records + lists)."""
+        """Surrogate pass (records + lists), with the exact solve when refine is set."""
+        self._network()
+        if self.params.refine == 2:
+            self.refine()
+
+    def refine(self) -> None:
+        """The Mino-time solve of the network's records (refine = 2 pass)."""
+        cam_arr = (abi.BH_Camera * len(self.sc.cams))(*self.sc.cams)
+        p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+        check(lib().bh_trace_disk_mino_refine(
+            cam_arr, len(self.sc.cams), C.byref(self.sc.opts), p(self.sc.luts), self.sc.lut_n, p(self.out),
+            p(self.fb_ids), p(self.counts[0:1]), self.fb_cap, self._stream()), "bh_trace_disk_mino_refine")
+
+    def _network(self) -> None:
         cam_arr = (abi.BH_Camera * len(self.sc.cams))(*self.sc.cams)
         L = S._sigs(lib())
         p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
@@ -90,7 +112,13 @@ class SurrogateRenderer:
         tracer.launch_ids(self.sc, self.fb_ids, self.counts[0:1], self.out)
 
     def audit(self) -> None:
-        tracer.launch_ids(self.sc, self.audit_ids, self.counts[1:2], self.audit_rec, compact=True)
+        self.audit_trace()
+        self.audit_compare()
+
+    def audit_trace(self, stream=None) -> None:
+        tracer.launch_ids(self.sc, self.audit_ids, self.counts[1:2], self.audit_rec, stream=stream, compact=True)
+
+    def audit_compare(self) -> None:
         L = S._sigs(lib())
         p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
         check(L.bh_surrogate_audit(p(self.out), p(self.audit_ids), p(self.counts[1:2]), self.audit_cap,
@@ -98,6 +126,21 @@ class SurrogateRenderer:
               "bh_surrogate_audit")
 
     def run(self, audit: bool = True) -> None:
+        if self.params.refine == 2 and audit:
+            torch = self.torch
+            main = torch.cuda.current_stream()
+            self._network()
+            listed = torch.cuda.Event()
+            listed.record(main)
+            self.side.wait_event(listed)
+            self.audit_trace(stream=self.side)
+            traced = torch.cuda.Event()
+            traced.record(self.side)
+            self.refine()
+            self.fallback()
+            main.wait_event(traced)
+            self.audit_compare()
+            return
         self.render()
         self.fallback()
         if audit:

@@ -378,6 +378,18 @@ int bh_trace_disk_mino(const BH_Camera* cams, int n_cams, const BH_TraceOptions*
                        int64_t* fallback_ids, uint64_t* fallback_count, int64_t fallback_cap,
                        void* stream);
 
+/* The same solver over records written by bh_surrogate_render with
+ * refine = 2: every record not flagged BH_FLAG_FALLBACK is solved in place
+ * with its crossing radii and count as Newton starts (flags
+ * BH_FLAG_SURROGATE | BH_FLAG_REFINED); failures are flagged and appended
+ * to the render's fallback list (the count is not reset).  Separate from
+ * the render so that the audit sample can be traced on another stream
+ * meanwhile.  No counterpart in the reference. */
+int bh_trace_disk_mino_refine(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+                              const double* disk_luts, int disk_lut_n, BH_PixelRecord* out,
+                              int64_t* fallback_ids, uint64_t* fallback_count, int64_t fallback_cap,
+                              void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,36 @@ def test_refined_surrogate_render_matches_oracle():
     assert it_guess <= 4.0 and it_fresh <= 4.0
 
 
+def test_overlapped_audit_pipeline_equals_serial_pipeline():
+    """The pipeline with the audit sample traced on a second stream (network
+    pass with refine = 2, bh_trace_disk_mino_refine, join) gives the records,
+    list lengths and audit statistics of the one-stream pipeline, bit for
+    bit, over repeated runs."""
+    import torch
+    from blackhole_amd import surrogate_train as ST
+    w = workloads.c5(96, spins=(0.3, 0.998), inclinations=(30.0, 80.0))
+    print("[overlap] 4 x 96^2 rays", flush=True)
+    sc = tracer.upload_scene(w["cams"], w["opts"], w["luts"])
+    sur = ST.load(WEIGHTS)
+    A = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(audit_every=7, overlap_audit=True))
+    B = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(audit_every=7, overlap_audit=False))
+    assert A.side is not None and A.params.refine == 2 and B.side is None and B.params.refine == 1
+    B.run()
+    torch.cuda.synchronize()
+    want = B.out.cpu().numpy()
+    for rep in range(3):
+        A.run()
+        torch.cuda.synchronize()
+        assert np.array_equal(A.out.cpu().numpy(), want), rep
+        assert A.counts_host() == B.counts_host()
+        sa, sb = A.audit_stats(), B.audit_stats()
+        for k in ("n", "n_cmp", "class_diff", "max_rel"):
+            assert sa[k] == sb[k], k
+        for k in ("sum_rel", "sum_abs_dI", "sum_I_ref"):   # wave sums added atomically: order varies
+            assert abs(sa[k] - sb[k]) <= 1e-12 * max(abs(sb[k]), 1e-300), k
+    assert A.counts_host()[1] > 100
+
+
 def test_trained_network_improves_on_its_prior_and_on_an_untrained_network():
     """What the trained network gains over the method without it: its
     first-crossing radii (the network alone, before any refinement) against
