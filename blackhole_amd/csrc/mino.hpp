@@ -51,26 +51,50 @@ using std::fmin;
 using std::sqrt;
 #endif
 
+// 1 / sqrt(v), v > 0: on the device the hardware reciprocal square root
+// (v_rsq_f64) and one Newton step y (3/2 - v y^2 / 2), relative error about
+// 1e-13 -- far below the quadrature tolerance -- for a fraction of the
+// correctly rounded sqrt-and-divide sequence; on the host 1 / sqrt(v).
+MINO_HD double inv_sqrt(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(v);
+  return y * (1.5 - 0.5 * v * (y * y));
+#else
+  return 1.0 / sqrt(v);
+#endif
+}
+
+// sqrt(v), v >= 0: on the device v / sqrt(v) from inv_sqrt (0 for v = 0),
+// relative error about 1e-13 like inv_sqrt; on the host sqrt(v).
+MINO_HD double fast_sqrt(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return v > 0.0 ? v * inv_sqrt(v) : 0.0;
+#else
+  return sqrt(v);
+#endif
+}
+
 // Carlson's R_F(x, y, z) for x, y, z >= 0, at most one zero: duplication until
-// the arguments agree to 2.5e-3, then the fifth-order series
+// the arguments agree to 2.5e-3 (|A - x| < 2.5e-3 A for each, no division),
+// then the fifth-order series in X = 1 - x / A, Y, Z:
 // (1 - E2/10 + E3/14 + E2^2/24 - 3 E2 E3 / 44) / sqrt(A).
 MINO_HD double carlson_rf(double x, double y, double z) {
   double A = (x + y + z) * (1.0 / 3.0);
   for (int it = 0; it < 60; ++it) {
-    const double dx = 1.0 - x / A, dy = 1.0 - y / A, dz = 1.0 - z / A;
-    if (fmax(fabs(dx), fmax(fabs(dy), fabs(dz))) < 2.5e-3) {
-      const double X = dx, Y = dy, Z = -(dx + dy);
+    if (fmax(fabs(A - x), fmax(fabs(A - y), fabs(A - z))) < 2.5e-3 * A) {
+      const double iA = 1.0 / A;
+      const double X = (A - x) * iA, Y = (A - y) * iA, Z = -(X + Y);
       const double E2 = X * Y - Z * Z, E3 = X * Y * Z;
-      return (1.0 - E2 / 10.0 + E3 / 14.0 + E2 * E2 / 24.0 - 3.0 * E2 * E3 / 44.0) / sqrt(A);
+      return (1.0 - E2 * 0.1 + E3 * (1.0 / 14.0) + E2 * E2 * (1.0 / 24.0) - E2 * E3 * (3.0 / 44.0)) * inv_sqrt(A);
     }
-    const double sx = sqrt(x), sy = sqrt(y), sz = sqrt(z);
+    const double sx = fast_sqrt(x), sy = fast_sqrt(y), sz = fast_sqrt(z);
     const double lam = sx * (sy + sz) + sy * sz;
     x = 0.25 * (x + lam);
     y = 0.25 * (y + lam);
     z = 0.25 * (z + lam);
     A = (x + y + z) * (1.0 / 3.0);
   }
-  return 1.0 / sqrt(A);
+  return inv_sqrt(A);
 }
 
 // Mino times of the polar motion: the m-th equatorial crossing comes at
@@ -87,11 +111,11 @@ MINO_HD PolarTimes polar_times(double a, double xi, double eta, double u_o, doub
   if (!(eta > 0.0) || u_o == 0.0) return t;
   const double a2 = a * a;
   const double b = eta + xi * xi - a2;
-  const double disc = sqrt(b * b + 4.0 * a2 * eta);
+  const double disc = fast_sqrt(b * b + 4.0 * a2 * eta);
   const double wp = (b >= 0.0) ? 2.0 * eta / (b + disc) : (disc - b) / (2.0 * a2);
-  const double k = sqrt(eta / wp);
+  const double k = fast_sqrt(eta / wp);
   const double mpar = -a2 * wp * wp / eta;
-  double s = fabs(u_o) / sqrt(wp);
+  double s = fabs(u_o) / fast_sqrt(wp);
   if (s > 1.0) s = 1.0;
   const double s2 = s * s;
   const double Fo = s * carlson_rf(1.0 - s2, 1.0 - mpar * s2, 1.0);
@@ -105,8 +129,8 @@ MINO_HD PolarTimes polar_times(double a, double xi, double eta, double u_o, doub
 
 // Largest real root of R when R has real roots r3 <= r4 (has34 = 1), from the
 // largest root y0 of the resolvent 8y^3 + 8A y^2 + (2A^2 - 8C) y - B^2 = 0:
-// with z = sqrt(y0 / 2), r3,4 = z -+ sqrt(-A/2 - z^2 - B / (4z)) and
-// r1,2 = -z -+ sqrt(-A/2 - z^2 + B / (4z)).  r4 is polished by Newton on R.
+// with z = fast_sqrt(y0 / 2), r3,4 = z -+ fast_sqrt(-A/2 - z^2 - B / (4z)) and
+// r1,2 = -z -+ fast_sqrt(-A/2 - z^2 + B / (4z)).  r4 is polished by Newton on R.
 struct RadialRoots {
   double r3, r4;
   int has34;
@@ -124,10 +148,10 @@ MINO_HD RadialRoots radial_roots(double A, double B, double C) {
   double t;
   if (D > 0.0) {
     const double h = -0.5 * Q;
-    const double u1 = cbrt(h + (h >= 0.0 ? sqrt(D) : -sqrt(D)));
+    const double u1 = cbrt(h + (h >= 0.0 ? fast_sqrt(D) : -fast_sqrt(D)));
     t = (u1 != 0.0) ? u1 - P / (3.0 * u1) : 0.0;
   } else {
-    const double rp = sqrt(fmax(-P / 3.0, 0.0));
+    const double rp = fast_sqrt(fmax(-P / 3.0, 0.0));
     double c = (rp > 0.0) ? -Q / (2.0 * rp * rp * rp) : 0.0;
     c = fmin(fmax(c, -1.0), 1.0);
     t = 2.0 * rp * cos(acos(c) / 3.0);
@@ -139,10 +163,10 @@ MINO_HD RadialRoots radial_roots(double A, double B, double C) {
     if (dh != 0.0) y -= h / dh;
   }
   if (!(y > 0.0)) return o;
-  const double z = sqrt(0.5 * y);
+  const double z = fast_sqrt(0.5 * y);
   const double rad = -0.5 * A - z * z - B / (4.0 * z);
   if (rad < 0.0) return o;
-  const double sr = sqrt(rad);
+  const double sr = fast_sqrt(rad);
   double r4 = z + sr;
   for (int it = 0; it < 2; ++it) {
     const double f = quartic_R(A, B, C, r4);
@@ -167,19 +191,6 @@ struct Path {
   double w_end;  // stopping radius (plunging) or escape radius (turning)
   int turning;
 };
-
-// 1 / sqrt(v), v > 0: on the device the hardware reciprocal square root
-// (v_rsq_f64) and one Newton step y (3/2 - v y^2 / 2), relative error about
-// 1e-13 -- far below the quadrature tolerance -- for a fraction of the
-// correctly rounded sqrt-and-divide sequence; on the host 1 / sqrt(v).
-MINO_HD double inv_sqrt(double v) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  const double y = __builtin_amdgcn_rsq(v);
-  return y * (1.5 - 0.5 * v * (y * y));
-#else
-  return 1.0 / sqrt(v);
-#endif
-}
 
 MINO_HD double path_x(const Path& p, double w) {
   if (p.turning) {
@@ -280,9 +291,9 @@ MINO_HD int make_path(double a, double xi, double eta, double r_o, double r_stop
     p->q2 = p->B + x4 * p->q3;
     p->q1 = p->A + x4 * p->q2;
     p->q0 = x4 * p->q1;
-    p->s_o = sqrt(x4 - p->x_o);
+    p->s_o = fast_sqrt(x4 - p->x_o);
     const double x_esc = 1.0 / r_escape;
-    p->w_end = p->s_o + (x4 > x_esc ? sqrt(x4 - x_esc) : 0.0);
+    p->w_end = p->s_o + (x4 > x_esc ? fast_sqrt(x4 - x_esc) : 0.0);
   } else {
     p->x4 = 0.0;
     p->q3 = p->q2 = p->q1 = p->q0 = 0.0;
@@ -305,13 +316,14 @@ MINO_HD int solve_crossing(const Path& p, double G, double w_lo, double T_lo, do
     // no usable start: invert the cubic Hermite model of T on the bracket
     // (values and slopes f at both ends), from the linear interpolant
     const double L = hi - lo;
-    const double d0 = path_f(p, lo) * L, d1 = path_f(p, hi) * L, dT = Thi - Tlo;
-    const double target = (G - Tlo) / dT;
+    const double idT = 1.0 / (Thi - Tlo);
+    const double d0 = path_f(p, lo) * L * idT, d1 = path_f(p, hi) * L * idT;  // end slopes over the T range
+    const double target = (G - Tlo) * idT;
     double t = target;
     for (int k = 0; k < 4; ++k) {
       const double t2 = t * t, t3 = t2 * t;
-      const double h = (3.0 * t2 - 2.0 * t3) + ((t3 - 2.0 * t2 + t) * d0 + (t3 - t2) * d1) / dT;
-      const double dh = (6.0 * t - 6.0 * t2) + ((3.0 * t2 - 4.0 * t + 1.0) * d0 + (3.0 * t2 - 2.0 * t) * d1) / dT;
+      const double h = (3.0 * t2 - 2.0 * t3) + ((t3 - 2.0 * t2 + t) * d0 + (t3 - t2) * d1);
+      const double dh = (6.0 * t - 6.0 * t2) + ((3.0 * t2 - 4.0 * t + 1.0) * d0 + (3.0 * t2 - 2.0 * t) * d1);
       if (!(dh > 0.0)) break;
       t -= (h - target) / dh;
       t = fmin(fmax(t, 0.0), 1.0);
@@ -368,7 +380,7 @@ MINO_HD double w_of_r(const Path& p, double r, bool outbound) {
   const double x = 1.0 / r;
   if (p.turning) {
     if (!(x <= p.x4)) return -1.0;
-    const double s = sqrt(p.x4 - x);
+    const double s = fast_sqrt(p.x4 - x);
     return outbound ? p.s_o + s : p.s_o - s;
   }
   return x - p.x_o;
