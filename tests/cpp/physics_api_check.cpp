@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <vector>
 
 #include "bh_physics.hpp"
 
@@ -87,6 +88,26 @@ int main(int argc, char** argv) {
           {v[0], v[1], v[2], v[3]}, {v[4], v[5], v[6], v[7]}, {v[8], v[9], v[10], v[11], v[12], v[13]});
       std::printf("%.17g %.17g %.17g %.17g\n", o.i, o.q, o.u, o.v);
     }
+  } else if (!std::strcmp(what, "stokes_path")) {
+    // first line I Q U V; then segments jI jQ jU jV alphaI rhoV ds -> the state after each
+    // stokesStep, then integrateStokesPath over all segments
+    physics::StokesVector s0;
+    if (std::scanf("%lf %lf %lf %lf", &s0.i, &s0.q, &s0.u, &s0.v) != 4) return 1;
+    std::vector<physics::StokesEmission> em;
+    std::vector<physics::FaradayPropagation> pr;
+    physics::StokesEmission e;
+    physics::FaradayPropagation k;
+    while (std::scanf("%lf %lf %lf %lf %lf %lf %lf", &e.jI, &e.jQ, &e.jU, &e.jV, &k.alphaI, &k.rhoV, &k.dsCm) == 7) {
+      em.push_back(e);
+      pr.push_back(k);
+    }
+    physics::StokesVector s = s0;
+    for (size_t i = 0; i < em.size(); ++i) {
+      s = physics::stokesStep(s, em[i], pr[i].alphaI, pr[i].rhoV, pr[i].dsCm);
+      std::printf("%.17g %.17g %.17g %.17g\n", s.i, s.q, s.u, s.v);
+    }
+    const physics::StokesVector f = physics::integrateStokesPath(em, pr, s0);
+    std::printf("%.17g %.17g %.17g %.17g\n", f.i, f.q, f.u, f.v);
   } else if (!std::strcmp(what, "stokes_misc")) {
     // one line of scalar functions and vector helpers
     const physics::StokesVector s{2.0, 0.6, 0.8, 0.3};

@@ -190,6 +190,76 @@ def test_faraday_coefficients(prog):
     assert out[1, 1] < out[1, 0] and out[1, 2] >= 0.0
 
 
+def _stokes_path(prog, s0, segs):
+    rows = [list(s0)] + [list(x) for x in segs]
+    out = prog("stokes_path", rows)
+    return out[:-1], out[-1]
+
+
+def test_faraday_rotation_invariants_over_many_steps(prog):
+    """The reference's stokes_invariants_test.cpp cases: pure rotation keeps
+    I, V, sqrt(Q^2 + U^2) and I^2 - Q^2 - U^2 - V^2 over hundreds of steps,
+    and the EVPA advances by sum rho ds / 2."""
+    rng = np.random.default_rng(21)
+    rho = rng.normal(size=400) * 0.7
+    ds = rng.uniform(0.01, 0.5, 400)
+    s0 = (1.0, 0.6, 0.0, 0.8)  # fully polarised: 0.36 + 0.64 = 1
+    states, final = _stokes_path(prog, s0, [(0, 0, 0, 0, 0.0, r, d) for r, d in zip(rho, ds)])
+    I, Q, U, V = states.T
+    assert np.abs(I - 1.0).max() <= 1e-12 and np.abs(V - 0.8).max() <= 1e-12
+    assert np.abs(np.hypot(Q, U) - 0.6).max() <= 1e-10
+    assert np.abs(I * I - Q * Q - U * U - V * V).max() <= 1e-10
+    phi = float((rho * ds).sum())
+    evpa = 0.5 * math.atan2(U[-1], Q[-1])
+    assert abs(math.remainder(evpa - 0.5 * phi, math.pi)) <= 1e-10
+    np.testing.assert_allclose(final, states[-1], rtol=1e-12, atol=1e-15)   # integrateStokesPath
+
+
+def test_absorption_keeps_polarisation_fraction_and_angle(prog):
+    s0 = (2.0, 0.5, -0.7, 0.3)
+    states, _ = _stokes_path(prog, s0, [(0, 0, 0, 0, 0.4, 0.0, 0.3)] * 20)
+    f = math.exp(-0.4 * 0.3 * 20)
+    np.testing.assert_allclose(states[-1], np.array(s0) * f, rtol=1e-12)
+    frac = np.sqrt((states[:, 1:] ** 2).sum(1)) / states[:, 0]
+    assert np.abs(frac - math.sqrt(0.83) / 2.0).max() <= 1e-12
+    evpa = 0.5 * np.arctan2(states[:, 2], states[:, 1])
+    assert np.abs(evpa - 0.5 * math.atan2(-0.7, 0.5)).max() <= 1e-12
+
+
+def test_polarisation_bound_along_a_random_path(prog):
+    """Physical emission (|j_P| <= j_I, |j_V| small) and absorption keep
+    I^2 >= Q^2 + U^2 + V^2 at every step of a 200-segment path."""
+    rng = np.random.default_rng(22)
+    segs = []
+    for _ in range(200):
+        jI = rng.uniform(0, 2)
+        frac, chi = rng.uniform(0, 0.75), rng.uniform(-1.6, 1.6)
+        segs.append((jI, -frac * jI * math.cos(2 * chi), -frac * jI * math.sin(2 * chi), 0.1 * jI * rng.normal() * 0.1,
+                     rng.uniform(0, 1.5), rng.normal() * 2, rng.uniform(0.01, 0.4)))
+    states, _ = _stokes_path(prog, (0, 0, 0, 0), segs)
+    I, Q, U, V = states.T
+    assert np.all(I >= 0.0)
+    assert np.all(I * I + 1e-10 >= Q * Q + U * U + V * V)
+
+
+def test_full_step_identity_and_conversion(prog):
+    s0 = [1.0, 0.3, -0.4, 0.1]
+    same = prog("stokes_full", [s0 + [0, 0, 0, 0] + [0, 0, 0, 0, 0, 0.7]])[0]
+    np.testing.assert_allclose(same, s0, rtol=1e-15)
+    # Faraday conversion (rho_Q) turns linear (U) into circular polarisation
+    s = [1.0, 0.0, 0.5, 0.0]
+    for _ in range(20):
+        s = list(prog("stokes_full", [s + [0, 0, 0, 0] + [0, 0, 0, 0, 0.8, 0.05]])[0])
+    assert abs(s[3]) > 1e-3 and abs(s[0] - 1.0) <= 1e-12
+    # rotation-dominated and absorption-dominated limits of the exact step
+    o = step(prog, (0, 0, 0, 0), (1.0, 0.4, -0.2, 0), 0.0, 2.0, 0.9)
+    ic, is_ = math.sin(1.8) / 2.0, (1 - math.cos(1.8)) / 2.0
+    assert abs(o[1] - (0.4 * ic + 0.2 * is_)) <= 1e-14 and abs(o[2] - (0.4 * is_ - 0.2 * ic)) <= 1e-14
+    o = step(prog, (0, 0, 0, 0), (1.0, 0.4, -0.2, 0), 1.3, 0.0, 0.9)
+    w = (1 - math.exp(-1.3 * 0.9)) / 1.3
+    assert abs(o[1] - 0.4 * w) <= 1e-15 and abs(o[2] + 0.2 * w) <= 1e-15
+
+
 def test_stokes_step_matches_reference(prog, ref):
     """Random states in every regime against the reference's stokesStep.
     Where the reference takes its first-order step (alpha ds < 1e-4: I and V
