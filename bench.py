@@ -442,7 +442,7 @@ def bench_c3(args):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
                      "frac": achieved / 157.3, "traffic": None, "kernel": "k_trace_fp32<DP45, volume>",
                      "kernel_ms": res_legs["fp32"]["kernel_ms"], "flop_per_launch": f,
-                     "note": "FP32 peak assumes packed FP32, which this build disables; scalar FP32 issues at "
+                     "note": "FP32 peak assumes packed FP32, which the library is compiled without; scalar FP32 issues at "
                              "the 78.6 TF FP64 rate"},
         "fp32": res_legs["fp32"], "fp64": res_legs["fp64"],
         "fp32_vs_fp64": {k: v for k, v in acc.items() if k != "frames"},
@@ -533,7 +533,7 @@ def main() -> int:
                     help="also run one 512 MiB device copy (profiler counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: the tracer runs on one GPU only (asked for --gpus {args.gpus}); "
               "rays are independent and would shard by rows or cameras without exchange", file=sys.stderr)
         return 2
     import torch

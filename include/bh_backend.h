@@ -3,7 +3,7 @@
  * src/cuda/cuda_backend.h:33-103) on the render launch of bh_kernel_launch.h.
  *
  * An opaque handle owns a linear device framebuffer (float4 [width * height],
- * row-major from the top row) and the active kernel variant.  This build has
+ * row-major from the top row) and the active kernel variant.  The library has
  * no OpenGL interop: the GL texture ids passed to bhCudaInit / bhCudaResize /
  * bhCudaRegisterLut / bhCudaRegisterBackgroundTexture are recorded and
  * reported back (bhCudaGlTexture) but not mapped, and a frame stays in the
@@ -45,7 +45,7 @@ int bhCudaGetVariant(BH_CudaBackend* backend);
 /* cuda_backend.h:105-110 */
 void bhCudaShutdown(BH_CudaBackend* backend);
 
-/* No counterpart in the reference (this build has no GL blit): the device
+/* No counterpart in the reference (the library has no GL blit): the device
  * framebuffer and its size, a wait for the last frame, and the recorded GL
  * ids (slot -1 = the framebuffer texture, 0..7 the LUTs, 8 the background). */
 void* bhCudaFramebuffer(BH_CudaBackend* backend, int* width, int* height);
