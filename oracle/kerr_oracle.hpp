@@ -17,9 +17,17 @@
 // same code with a counting scalar (FMA = 2, +-*/ sqrt = 1, transcendentals
 // tallied separately) and fix the algorithmic FLOP per step.
 //
-// Pinning (tests/test_oracle_*.py): golden fixtures of the reference
-// (assets/luts/*.csv), known answers of the reference's tests, the
-// reference's own functions compiled into oracle/_ref, and closed forms.
+// Pinning (tests/test_oracle_pinning.py, tests/test_stokes.py): golden
+// fixtures of the reference (assets/luts/*.csv), known answers of the
+// reference's tests, the reference's own functions compiled into
+// oracle/_ref, and closed forms.  Every function here has at least one such
+// check.  The RK4 / DP45 trajectories have no reference vector (the
+// reference's stepper stalls at turning points and its DP45 does not
+// compile); they are pinned by closed forms instead: the Schwarzschild
+// capture threshold and deflection integral, the Kerr radial turning point,
+// the conserved integrals and the local error orders.  pol_segment is pinned
+// by composing the reference's compiled stokesStep forward along random
+// paths.
 #pragma once
 
 #include <cmath>
