@@ -9,6 +9,7 @@
 #include "physics/kerr.h"
 #include "physics/rte_integrator.h"
 #include "physics/stokes_transport.h"
+#include "physics/symplectic_integrator.h"
 #include "physics/verified/geodesic.hpp"
 
 namespace {
@@ -127,5 +128,39 @@ double ref_faraday_rotation_coeff(double nu, double ne, double b) {
 double ref_synch_lin_pol_frac(double p) { return physics::synchrotronLinearPolarFrac(p); }
 
 double ref_thermal_lin_pol_frac(double theta_e) { return physics::thermalSynchLinearPolarFrac(theta_e); }
+
+// symplectic_integrator.h: st8 = (t, r, theta, phi, p_t, p_r, p_theta, p_phi).
+static physics::GeodesicState ref_state(const double* st8) {
+  physics::GeodesicState s;
+  for (int i = 0; i < 4; ++i) {
+    s.q[i] = st8[i];
+    s.p[i] = st8[4 + i];
+  }
+  return s;
+}
+
+void ref_kerr_gcon(double r, double theta, double a, double rS, double* out16) {
+  const physics::Metric4 g = physics::kerrGcon(r, theta, a, rS);
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) out16[4 * i + j] = g[i][j];
+}
+
+double ref_kerr_hamiltonian(const double* st8, double a, double rS) {
+  return physics::kerrHamiltonian(ref_state(st8), physics::KerrParams{a, rS});
+}
+
+void ref_kerr_force(const double* st8, double a, double rS, double* out4) {
+  const std::array<double, 4> f = physics::kerrForce(ref_state(st8), physics::KerrParams{a, rS});
+  for (int i = 0; i < 4; ++i) out4[i] = f[i];
+}
+
+void ref_yoshida4(double* st8, double a, double rS, double h, int n) {
+  physics::GeodesicState s = ref_state(st8);
+  for (int k = 0; k < n; ++k) physics::yoshida4Step(s, h, physics::KerrParams{a, rS});
+  for (int i = 0; i < 4; ++i) {
+    st8[i] = s.q[i];
+    st8[4 + i] = s.p[i];
+  }
+}
 
 }  // extern "C"

@@ -88,6 +88,35 @@ int main(int argc, char** argv) {
           {v[0], v[1], v[2], v[3]}, {v[4], v[5], v[6], v[7]}, {v[8], v[9], v[10], v[11], v[12], v[13]});
       std::printf("%.17g %.17g %.17g %.17g\n", o.i, o.q, o.u, o.v);
     }
+  } else if (!std::strcmp(what, "hamilton")) {
+    // a rS t r theta phi p_t p_r p_theta p_phi -> g^{mu nu} (16), H, velocity (4), force (4)
+    double v[10];
+    while (std::scanf("%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", v, v + 1, v + 2, v + 3, v + 4, v + 5, v + 6, v + 7,
+                      v + 8, v + 9) == 10) {
+      const physics::KerrParams k{v[0], v[1]};
+      const physics::GeodesicState st{{v[2], v[3], v[4], v[5]}, {v[6], v[7], v[8], v[9]}};
+      const physics::Metric4 g = physics::kerrGcon(v[3], v[4], v[0], v[1]);
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) std::printf("%.17g ", g[i][j]);
+      std::printf("%.17g ", physics::kerrHamiltonian(st, k));
+      const auto vel = physics::kerrVelocity(st, k);
+      const auto f = physics::kerrForce(st, k);
+      for (int i = 0; i < 4; ++i) std::printf("%.17g ", vel[i]);
+      for (int i = 0; i < 4; ++i) std::printf("%.17g%s", f[i], i == 3 ? "\n" : " ");
+    }
+  } else if (!std::strcmp(what, "yoshida")) {
+    // a rS h n t r theta phi p_t p_r p_theta p_phi -> the state after n steps and H
+    double v[12];
+    while (std::scanf("%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", v, v + 1, v + 2, v + 3, v + 4, v + 5,
+                      v + 6, v + 7, v + 8, v + 9, v + 10, v + 11) == 12) {
+      const physics::KerrParams k{v[0], v[1]};
+      physics::GeodesicState st{{v[4], v[5], v[6], v[7]}, {v[8], v[9], v[10], v[11]}};
+      const int n = static_cast<int>(v[3]);
+      for (int i = 0; i < n; ++i) physics::yoshida4Step(st, v[2], k);
+      for (int i = 0; i < 4; ++i) std::printf("%.17g ", st.q[i]);
+      for (int i = 0; i < 4; ++i) std::printf("%.17g ", st.p[i]);
+      std::printf("%.17g\n", physics::kerrHamiltonian(st, k));
+    }
   } else if (!std::strcmp(what, "stokes_path")) {
     // first line I Q U V; then segments jI jQ jU jV alphaI rhoV ds -> the state after each
     // stokesStep, then integrateStokesPath over all segments
