@@ -1,6 +1,7 @@
 // bh_physics.hpp - the reference's physics C++ interface for the geodesic
 // path (src/physics/kerr.h:91-546, src/physics/raytracer.h:46-79,309-396,
-// src/physics/verified/rk4.hpp:49-437), its Hamiltonian form
+// src/physics/verified/rk4.hpp:49-437), its event detection
+// (src/physics/event_detection.h:39-241), its Hamiltonian form
 // (src/physics/symplectic_integrator.h:40-298, connection.h:127-178) and
 // polarised transfer (src/physics/stokes_transport.h:130-747), header-only
 // C++17 over the library.
@@ -351,6 +352,95 @@ class KerrRaytracer {
   int maxSteps_{10000};
   bool recordPath_{false};
 };
+
+// ---------------------------------------------------------------------------
+// Event detection along a ray (src/physics/event_detection.h:39-241): sign
+// changes of r - r_target, v_r or theta - pi/2 between two samples, and
+// bisection of the affine parameter to refine a radial crossing or a radial
+// turning point.  State is any type with members x1 (r), x2 (theta) and v1
+// (dr/dlambda); stepper(l_a, l_b, s_a) returns the state at l_b from s_a.
+// ---------------------------------------------------------------------------
+
+enum class EventType { None, HorizonCrossing, PhotonSphereCrossing, TurningPoint, RadialThreshold, EquatorialCrossing };
+
+struct EventResult {
+  EventType type = EventType::None;
+  double lambda = 0.0;  // affine parameter at the event
+  double r = 0.0;
+  double theta = 0.0;
+  int nBisections = 0;
+  bool detected = false;
+};
+
+[[nodiscard]] inline bool crossedRadius(double r0, double r1, double rTarget) {
+  return (r0 - rTarget) * (r1 - rTarget) < 0.0;
+}
+[[nodiscard]] inline bool hasTurningPoint(double vr0, double vr1) { return vr0 * vr1 < 0.0; }
+[[nodiscard]] inline bool crossedEquator(double theta0, double theta1) {
+  return (theta0 - 0.5 * PI) * (theta1 - 0.5 * PI) < 0.0;
+}
+
+namespace detail {
+// Halves [l0, l1] while keeping the sign change of f(state) in the kept half
+// (the left end's state is carried, so the stepper always starts there);
+// stops when |f(mid)| < tol or after max_iter halvings.
+template <typename State, typename Stepper, typename F>
+EventResult bisect_event(EventType type, double l0, double l1, const State& s0, const Stepper& stepper, F f,
+                         double tol, int max_iter) {
+  EventResult res;
+  res.type = type;
+  double la = l0, lb = l1;
+  State sa = s0;
+  for (int i = 0; i < max_iter; ++i) {
+    const double lm = 0.5 * (la + lb);
+    const State sm = stepper(la, lm, sa);
+    if (std::fabs(f(sm)) < tol) {
+      res.detected = true;
+      res.lambda = lm;
+      res.r = sm.x1;
+      res.theta = sm.x2;
+      res.nBisections = i + 1;
+      return res;
+    }
+    if (f(sa) * f(sm) < 0.0) {
+      lb = lm;
+    } else {
+      la = lm;
+      sa = sm;
+    }
+  }
+  const double lm = 0.5 * (la + lb);
+  const State sm = stepper(la, lm, sa);
+  res.detected = true;  // best estimate after max_iter halvings
+  res.lambda = lm;
+  res.r = sm.x1;
+  res.theta = sm.x2;
+  res.nBisections = max_iter;
+  return res;
+}
+}  // namespace detail
+
+// The affine parameter where r passes rTarget within [lambda0, lambda1]
+// (event_detection.h:127-191).
+template <typename State, typename Stepper>
+EventResult bisectRadialCrossing(double lambda0, double lambda1, const State& state0, double rTarget,
+                                 const Stepper& stepper, double tolR = 1e-10, int maxIter = 50) {
+  return detail::bisect_event(
+      EventType::RadialThreshold, lambda0, lambda1, state0, stepper,
+      [rTarget](const State& s) { return s.x1 - rTarget; }, tolR, maxIter);
+}
+
+// The affine parameter where v_r changes sign within [lambda0, lambda1].
+template <typename State, typename Stepper>
+EventResult bisectTurningPoint(double lambda0, double lambda1, const State& state0, const Stepper& stepper,
+                               double tolVr = 1e-12, int maxIter = 50) {
+  return detail::bisect_event(
+      EventType::TurningPoint, lambda0, lambda1, state0, stepper, [](const State& s) { return s.v1; }, tolVr,
+      maxIter);
+}
+
+// The n-th photon subring is labelled by its number of radial turning points.
+[[nodiscard]] inline int photonSubringIndex(int turningPointCount) { return turningPointCount; }
 
 // ---------------------------------------------------------------------------
 // Hamiltonian form (src/physics/symplectic_integrator.h:40-298 and the

@@ -10,6 +10,7 @@
 #include "physics/rte_integrator.h"
 #include "physics/stokes_transport.h"
 #include "physics/symplectic_integrator.h"
+#include "physics/event_detection.h"
 #include "physics/verified/geodesic.hpp"
 
 namespace {
@@ -128,6 +129,40 @@ double ref_faraday_rotation_coeff(double nu, double ne, double b) {
 double ref_synch_lin_pol_frac(double p) { return physics::synchrotronLinearPolarFrac(p); }
 
 double ref_thermal_lin_pol_frac(double theta_e) { return physics::thermalSynchLinearPolarFrac(theta_e); }
+
+// event_detection.h on the analytic rays r = 12 - 0.7 l - 0.01 l^3 (through
+// r = 8) and r = 10 + (l - 2.3)^2 (turning at 2.3); out = 3 x (type, lambda,
+// r, theta, nBisections, detected) then the predicates and the subring index.
+void ref_events(double* out) {
+  struct St {
+    double x1, x2, v1;
+  };
+  auto ray1 = [](double, double lb, const St&) {
+    return St{12.0 - 0.7 * lb - 0.01 * lb * lb * lb, 1.0 + 0.1 * lb, -0.7 - 0.03 * lb * lb};
+  };
+  auto ray2 = [](double, double lb, const St&) {
+    return St{10.0 + (lb - 2.3) * (lb - 2.3), 1.2 - 0.05 * lb, 2.0 * (lb - 2.3)};
+  };
+  const physics::EventResult e[3] = {
+      physics::bisectRadialCrossing(0.0, 10.0, St{12.0, 1.0, -0.7}, 8.0, ray1),
+      physics::bisectTurningPoint(0.0, 5.0, St{10.0 + 2.3 * 2.3, 1.2, -4.6}, ray2),
+      physics::bisectRadialCrossing(0.0, 10.0, St{12.0, 1.0, -0.7}, 8.0, ray1, 1e-30, 7)};
+  for (int i = 0; i < 3; ++i) {
+    out[6 * i] = static_cast<double>(static_cast<int>(e[i].type));
+    out[6 * i + 1] = e[i].lambda;
+    out[6 * i + 2] = e[i].r;
+    out[6 * i + 3] = e[i].theta;
+    out[6 * i + 4] = e[i].nBisections;
+    out[6 * i + 5] = e[i].detected ? 1.0 : 0.0;
+  }
+  out[18] = physics::crossedRadius(5.0, 7.0, 6.0);
+  out[19] = physics::crossedRadius(5.0, 7.0, 8.0);
+  out[20] = physics::hasTurningPoint(-1.0, 0.5);
+  out[21] = physics::hasTurningPoint(0.2, 0.5);
+  out[22] = physics::crossedEquator(1.5, 1.6);
+  out[23] = physics::crossedEquator(1.0, 1.5);
+  out[24] = physics::photonSubringIndex(3);
+}
 
 // symplectic_integrator.h: st8 = (t, r, theta, phi, p_t, p_r, p_theta, p_phi).
 static physics::GeodesicState ref_state(const double* st8) {

@@ -88,6 +88,26 @@ int main(int argc, char** argv) {
           {v[0], v[1], v[2], v[3]}, {v[4], v[5], v[6], v[7]}, {v[8], v[9], v[10], v[11], v[12], v[13]});
       std::printf("%.17g %.17g %.17g %.17g\n", o.i, o.q, o.u, o.v);
     }
+  } else if (!std::strcmp(what, "events")) {
+    // analytic rays: r = 12 - 0.7 l - 0.01 l^3 through r = 8; r = 10 + (l - 2.3)^2 turning at 2.3
+    struct St {
+      double x1, x2, v1;
+    };
+    auto ray1 = [](double, double lb, const St&) {
+      return St{12.0 - 0.7 * lb - 0.01 * lb * lb * lb, 1.0 + 0.1 * lb, -0.7 - 0.03 * lb * lb};
+    };
+    auto ray2 = [](double, double lb, const St&) {
+      return St{10.0 + (lb - 2.3) * (lb - 2.3), 1.2 - 0.05 * lb, 2.0 * (lb - 2.3)};
+    };
+    const physics::EventResult e1 = physics::bisectRadialCrossing(0.0, 10.0, St{12.0, 1.0, -0.7}, 8.0, ray1);
+    const physics::EventResult e2 = physics::bisectTurningPoint(0.0, 5.0, St{10.0 + 2.3 * 2.3, 1.2, -4.6}, ray2);
+    const physics::EventResult e3 = physics::bisectRadialCrossing(0.0, 10.0, St{12.0, 1.0, -0.7}, 8.0, ray1, 1e-30, 7);
+    for (const physics::EventResult& e : {e1, e2, e3})
+      std::printf("%d %.17g %.17g %.17g %d %d\n", static_cast<int>(e.type), e.lambda, e.r, e.theta, e.nBisections,
+                  e.detected ? 1 : 0);
+    std::printf("%d %d %d %d %d %d %d\n", physics::crossedRadius(5.0, 7.0, 6.0), physics::crossedRadius(5.0, 7.0, 8.0),
+                physics::hasTurningPoint(-1.0, 0.5), physics::hasTurningPoint(0.2, 0.5),
+                physics::crossedEquator(1.5, 1.6), physics::crossedEquator(1.0, 1.5), physics::photonSubringIndex(3));
   } else if (!std::strcmp(what, "hamilton")) {
     // a rS t r theta phi p_t p_r p_theta p_phi -> g^{mu nu} (16), H, velocity (4), force (4)
     double v[10];

@@ -1,5 +1,7 @@
-"""The Hamiltonian form of the physics interface (include/bh_physics.hpp:
-kerrGcon, kerrHamiltonian, kerrVelocity, kerrForce, yoshida4Step) on the CPU.
+"""The Hamiltonian form (kerrGcon, kerrHamiltonian, kerrVelocity, kerrForce,
+yoshida4Step) and the event detection (crossedRadius, hasTurningPoint,
+crossedEquator, bisectRadialCrossing, bisectTurningPoint, photonSubringIndex)
+of the physics interface (include/bh_physics.hpp) on the CPU.
 
 Pinned by the reference's own functions compiled from
 src/physics/symplectic_integrator.h and connection.h into oracle/_ref
@@ -41,7 +43,8 @@ def prog(tmp_path_factory):
     def run(what, rows):
         stdin = "\n".join(" ".join(f"{v:.17g}" for v in r) for r in rows)
         out = subprocess.run([str(exe), what], input=stdin, capture_output=True, text=True, check=True).stdout
-        return np.array([[float(v) for v in l.split()] for l in out.strip().split("\n")])
+        rows = [np.array([float(v) for v in l.split()]) for l in out.strip().split("\n")]
+        return np.array(rows) if len({len(r) for r in rows}) == 1 else rows
     return run
 
 
@@ -152,3 +155,28 @@ def test_yoshida_composition_matches_reference_and_is_first_order(prog, ref):
     dr = [abs(res[0.4][1] - res[0.2][1]), abs(res[0.2][1] - res[0.1][1])]
     assert 1.8 < dH[0] / dH[1] < 2.2 and 1.8 < dH[1] / dH[2] < 2.2     # first order in h
     assert 1.8 < dr[0] / dr[1] < 2.2
+
+
+def test_event_detection(prog):
+    """Bisection on analytic rays finds the crossing of r = 8 by
+    r = 12 - 0.7 l - 0.01 l^3 and the turning point of r = 10 + (l - 2.3)^2
+    to their tolerances; the predicates and the subring index."""
+    rows = prog("events", [[0]])
+    e1, e2, e3 = rows[0], rows[1], rows[2]
+    # l* solves 0.01 l^3 + 0.7 l - 4 = 0
+    ls = np.roots([0.01, 0.0, 0.7, -4.0])
+    ls = float(ls[np.abs(ls.imag) < 1e-12].real[0])
+    assert e1[0] == 4 and e1[5] == 1 and abs(e1[2] - 8.0) < 1e-10 and abs(e1[1] - ls) < 2e-10
+    assert abs(e1[3] - (1.0 + 0.1 * e1[1])) < 1e-15
+    assert e2[0] == 3 and e2[5] == 1 and abs(e2[1] - 2.3) <= 1e-12
+    assert e3[4] == 7 and abs(e3[1] - ls) <= 10.0 / 2 ** 8        # budget: 7 halvings, then the midpoint
+    assert list(rows[3]) == [1, 0, 1, 0, 1, 0, 3]
+
+
+def test_event_detection_matches_reference(prog, ref):
+    ref.ref_events.argtypes = [_PD]
+    want = np.zeros(25)
+    ref.ref_events(P(want))
+    rows = prog("events", [[0]])
+    got = np.concatenate([rows[0], rows[1], rows[2], rows[3]])
+    np.testing.assert_array_equal(got, want)
