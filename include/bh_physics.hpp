@@ -1,6 +1,7 @@
 // bh_physics.hpp - the reference's physics C++ interface for the geodesic
 // path (src/physics/kerr.h:91-546, src/physics/raytracer.h:46-79,309-396,
-// src/physics/verified/rk4.hpp:49-437), its event detection
+// src/physics/verified/rk4.hpp:49-437), its conserved quantities
+// (src/physics/conservation_monitor.h:30-184), event detection
 // (src/physics/event_detection.h:39-241), its Hamiltonian form
 // (src/physics/symplectic_integrator.h:40-298, connection.h:127-178) and
 // polarised transfer (src/physics/stokes_transport.h:130-747), header-only
@@ -354,6 +355,72 @@ class KerrRaytracer {
 };
 
 // ---------------------------------------------------------------------------
+// Conserved quantities of a geodesic from its contravariant velocity
+// (src/physics/conservation_monitor.h:30-184), geometric units, rS = 2 mGeom:
+// E = -u_t, L_z = u_phi, mu2 = -g(u, u) (0 for photons), Carter
+// Q = u_theta^2 + cos^2 theta (a^2 (mu2 - E^2) + L_z^2 / sin^2 theta).
+// ---------------------------------------------------------------------------
+
+struct ConservedQuantities {
+  double e = 0.0, lz = 0.0, q = 0.0, mu2 = 0.0;
+};
+
+[[nodiscard]] inline ConservedQuantities computeConserved(double r, double theta, double ut, double ur,
+                                                          double utheta, double uphi, double mGeom, double a) {
+  const double s = std::sin(theta), c = std::cos(theta), s2 = s * s, c2 = c * c;
+  const double rS = 2.0 * mGeom;
+  const double sig = r * r + a * a * c2, del = r * r - rS * r + a * a;
+  // covariant metric components (Boyer-Lindquist)
+  const double g_tt = -(1.0 - rS * r / sig);
+  const double g_tp = -rS * r * a * s2 / sig;
+  const double g_pp = (r * r + a * a + rS * r * a * a * s2 / sig) * s2;
+  const double g_rr = sig / del, g_hh = sig;
+  ConservedQuantities cq;
+  const double u_t = g_tt * ut + g_tp * uphi, u_p = g_tp * ut + g_pp * uphi, u_h = g_hh * utheta;
+  cq.e = -u_t;
+  cq.lz = u_p;
+  cq.mu2 = -(u_t * ut + u_p * uphi + g_rr * ur * ur + u_h * utheta);
+  cq.q = u_h * u_h + c2 * (a * a * (cq.mu2 - cq.e * cq.e) + cq.lz * cq.lz / s2);
+  return cq;
+}
+
+// Largest drift of E, L_z, Q (relative to the first state, floored at 1e-30)
+// and of mu2 (absolute) over the states it has been updated with.
+struct GeodesicConservationMonitor {
+  ConservedQuantities initial;
+  bool initialized = false;
+  double maxEDrift = 0.0, maxLzDrift = 0.0, maxQDrift = 0.0, maxMu2Drift = 0.0;
+  int stepCount = 0;
+
+  void init(const ConservedQuantities& cq) {
+    *this = GeodesicConservationMonitor{};
+    initial = cq;
+    initialized = true;
+  }
+  // Returns the largest of the current drifts (0 on the first call, which
+  // records the initial state).
+  [[nodiscard]] double update(const ConservedQuantities& cur) {
+    if (!initialized) {
+      init(cur);
+      return 0.0;
+    }
+    ++stepCount;
+    auto rel = [](double v0, double v1) { return std::fabs(v1 - v0) / std::max(std::fabs(v0), 1e-30); };
+    const double dE = rel(initial.e, cur.e), dL = rel(initial.lz, cur.lz), dQ = rel(initial.q, cur.q);
+    const double dM = std::fabs(cur.mu2 - initial.mu2);
+    maxEDrift = std::max(maxEDrift, dE);
+    maxLzDrift = std::max(maxLzDrift, dL);
+    maxQDrift = std::max(maxQDrift, dQ);
+    maxMu2Drift = std::max(maxMu2Drift, dM);
+    return std::max(std::max(dE, dL), std::max(dQ, dM));
+  }
+  [[nodiscard]] bool violated(double threshold = 1e-8) const { return maxDrift() > threshold; }
+  [[nodiscard]] double maxDrift() const {
+    return std::max(std::max(maxEDrift, maxLzDrift), std::max(maxQDrift, maxMu2Drift));
+  }
+};
+
+// ---------------------------------------------------------------------------
 // Event detection along a ray (src/physics/event_detection.h:39-241): sign
 // changes of r - r_target, v_r or theta - pi/2 between two samples, and
 // bisection of the affine parameter to refine a radial crossing or a radial
@@ -557,6 +624,14 @@ inline void yoshida4Step(GeodesicState& s, double h, const KerrParams& k) noexce
   applyDrift(s, c2 * h, k);
   applyKick(s, w1 * h, k);
   applyDrift(s, c1 * h, k);
+}
+
+// nSteps Yoshida steps of dLambda / nSteps (symplectic_integrator.h:304-311);
+// a negative dLambda integrates backwards.
+inline void integrateGeodesic(GeodesicState& s, double dLambda, int nSteps, const KerrParams& k) noexcept {
+  if (nSteps <= 0) return;
+  const double h = dLambda / static_cast<double>(nSteps);
+  for (int i = 0; i < nSteps; ++i) yoshida4Step(s, h, k);
 }
 
 // ---------------------------------------------------------------------------

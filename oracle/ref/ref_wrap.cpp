@@ -11,6 +11,7 @@
 #include "physics/stokes_transport.h"
 #include "physics/symplectic_integrator.h"
 #include "physics/event_detection.h"
+#include "physics/conservation_monitor.h"
 #include "physics/verified/geodesic.hpp"
 
 namespace {
@@ -129,6 +130,16 @@ double ref_faraday_rotation_coeff(double nu, double ne, double b) {
 double ref_synch_lin_pol_frac(double p) { return physics::synchrotronLinearPolarFrac(p); }
 
 double ref_thermal_lin_pol_frac(double theta_e) { return physics::thermalSynchLinearPolarFrac(theta_e); }
+
+// conservation_monitor.h: out4 = (e, lz, q, mu2).
+void ref_compute_conserved(double r, double theta, double ut, double ur, double uth, double uph, double m,
+                           double a, double* out4) {
+  const physics::ConservedQuantities c = physics::computeConserved(r, theta, ut, ur, uth, uph, m, a);
+  out4[0] = c.e;
+  out4[1] = c.lz;
+  out4[2] = c.q;
+  out4[3] = c.mu2;
+}
 
 // event_detection.h on the analytic rays r = 12 - 0.7 l - 0.01 l^3 (through
 // r = 8) and r = 10 + (l - 2.3)^2 (turning at 2.3); out = 3 x (type, lambda,

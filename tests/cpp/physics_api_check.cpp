@@ -88,6 +88,18 @@ int main(int argc, char** argv) {
           {v[0], v[1], v[2], v[3]}, {v[4], v[5], v[6], v[7]}, {v[8], v[9], v[10], v[11], v[12], v[13]});
       std::printf("%.17g %.17g %.17g %.17g\n", o.i, o.q, o.u, o.v);
     }
+  } else if (!std::strcmp(what, "conserved")) {
+    // r theta ut ur utheta uphi mGeom a -> e lz q mu2; the monitor's drifts over the rows
+    double v[8];
+    physics::GeodesicConservationMonitor mon;
+    double last = 0.0;
+    while (std::scanf("%lf %lf %lf %lf %lf %lf %lf %lf", v, v + 1, v + 2, v + 3, v + 4, v + 5, v + 6, v + 7) == 8) {
+      const physics::ConservedQuantities c = physics::computeConserved(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
+      last = mon.update(c);
+      std::printf("%.17g %.17g %.17g %.17g\n", c.e, c.lz, c.q, c.mu2);
+    }
+    std::printf("%.17g %.17g %.17g %.17g %.17g %d %d\n", last, mon.maxEDrift, mon.maxLzDrift, mon.maxQDrift,
+                mon.maxMu2Drift, mon.stepCount, mon.violated(1e-8) ? 1 : 0);
   } else if (!std::strcmp(what, "events")) {
     // analytic rays: r = 12 - 0.7 l - 0.01 l^3 through r = 8; r = 10 + (l - 2.3)^2 turning at 2.3
     struct St {
@@ -123,6 +135,18 @@ int main(int argc, char** argv) {
       const auto f = physics::kerrForce(st, k);
       for (int i = 0; i < 4; ++i) std::printf("%.17g ", vel[i]);
       for (int i = 0; i < 4; ++i) std::printf("%.17g%s", f[i], i == 3 ? "\n" : " ");
+    }
+  } else if (!std::strcmp(what, "integrate")) {
+    // a rS dLambda n t r theta phi p_t p_r p_theta p_phi -> integrateGeodesic state and H
+    double v[12];
+    while (std::scanf("%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", v, v + 1, v + 2, v + 3, v + 4, v + 5,
+                      v + 6, v + 7, v + 8, v + 9, v + 10, v + 11) == 12) {
+      const physics::KerrParams k{v[0], v[1]};
+      physics::GeodesicState st{{v[4], v[5], v[6], v[7]}, {v[8], v[9], v[10], v[11]}};
+      physics::integrateGeodesic(st, v[2], static_cast<int>(v[3]), k);
+      for (int i = 0; i < 4; ++i) std::printf("%.17g ", st.q[i]);
+      for (int i = 0; i < 4; ++i) std::printf("%.17g ", st.p[i]);
+      std::printf("%.17g\n", physics::kerrHamiltonian(st, k));
     }
   } else if (!std::strcmp(what, "yoshida")) {
     // a rS h n t r theta phi p_t p_r p_theta p_phi -> the state after n steps and H

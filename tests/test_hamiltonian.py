@@ -1,5 +1,6 @@
 """The Hamiltonian form (kerrGcon, kerrHamiltonian, kerrVelocity, kerrForce,
-yoshida4Step) and the event detection (crossedRadius, hasTurningPoint,
+yoshida4Step, integrateGeodesic), the conserved quantities (computeConserved,
+GeodesicConservationMonitor) and the event detection (crossedRadius, hasTurningPoint,
 crossedEquator, bisectRadialCrossing, bisectTurningPoint, photonSubringIndex)
 of the physics interface (include/bh_physics.hpp) on the CPU.
 
@@ -180,3 +181,66 @@ def test_event_detection_matches_reference(prog, ref):
     rows = prog("events", [[0]])
     got = np.concatenate([rows[0], rows[1], rows[2], rows[3]])
     np.testing.assert_array_equal(got, want)
+
+
+def test_reference_symplectic_known_answers(prog):
+    """tests/symplectic_integrator_test.cpp: the timelike orbit from r = 10 M
+    (a = 0.5, p = (-1, 0, 0, 4.878)) has H near -1/2; 1000 steps of h = 0.01
+    keep |H_N - H_0| < 1e-6 and the cyclic momenta to 1e-10; 500 steps
+    forward then 500 back return to the start within 1e-4."""
+    st = [0.0, 10.0, math.pi / 2, 0.0, -1.0, 0.0, 0.0, 4.878]
+    a, rS = 0.5, 2.0
+    H0 = prog("hamilton", [[a, rS] + st])[0][16]
+    assert abs(H0 + 0.5) <= 5e-3 and H0 < 0.0
+    out = prog("integrate", [[a, rS, 10.0, 1000] + st])[0]
+    assert abs(out[8] - H0) < 1e-6, out[8] - H0
+    assert abs(out[4] - st[4]) < 1e-10 and abs(out[7] - st[7]) < 1e-10
+    fwd = prog("integrate", [[a, rS, 5.0, 500] + st])[0]
+    back = prog("integrate", [[a, rS, -5.0, 500] + list(fwd[:8])])[0]
+    assert np.abs(back[:8] - np.array(st)).max() < 1e-4
+    assert abs(back[4] - st[4]) < 1e-14 and abs(back[7] - st[7]) < 1e-14
+
+
+def _null_velocity(a, r, th, xi, eta, sr=-1.0, sth=1.0):
+    """Contravariant velocity u^mu = g^{mu nu} p_nu of the photon (E = 1,
+    L = xi, Carter eta) at (r, theta)."""
+    st = null_state(a, r, th, xi, eta, inbound=sr < 0, up=sth > 0)
+    s2, c2 = math.sin(th) ** 2, math.cos(th) ** 2
+    sig, dl = r * r + a * a * c2, r * r - 2 * r + a * a
+    A = (r * r + a * a) ** 2 - a * a * dl * s2
+    pt, pr, pth, pph = st[4:]
+    ut = -A / (dl * sig) * pt - 2 * r * a / (dl * sig) * pph
+    uph = -2 * r * a / (dl * sig) * pt + (dl - a * a * s2) / (dl * sig * s2) * pph
+    return [r, th, ut, dl / sig * pr, pth / sig, uph]
+
+
+def test_conserved_quantities_round_trip_and_monitor(prog):
+    """A photon built from (E, L, Q) = (1, xi, eta) gives them back, with
+    mu2 = 0; the monitor's drift bookkeeping."""
+    cases = ((0.9, 20.0, 1.0, 3.0, 20.0), (0.5, 8.0, 0.6, -2.0, 12.0), (0.0, 30.0, 1.4, 5.0, 4.0))
+    rows = [_null_velocity(a, r, th, xi, eta) + [1.0, a] for a, r, th, xi, eta in cases]
+    out = prog("conserved", rows)
+    for (a, r, th, xi, eta), o in zip(cases, out[:3]):
+        assert abs(o[0] - 1.0) <= 1e-12 and abs(o[1] - xi) <= 1e-12 * max(1, abs(xi))
+        assert abs(o[2] - eta) <= 1e-10 * eta and abs(o[3]) <= 1e-12
+    # the monitor: first row initial, later rows drift relative to it
+    base = _null_velocity(0.9, 20.0, 1.0, 3.0, 20.0) + [1.0, 0.9]
+    moved = list(base)
+    moved[2] *= 1.0 + 1e-6     # u^t off by 1e-6: E, L, mu2 drift
+    out = prog("conserved", [base, base, moved])
+    last, dE, dL, dQ, dM, steps, viol = out[-1]
+    assert steps == 2 and viol == 1 and dE > 1e-7 and last == max(dE, dL, dQ, dM)
+
+
+def test_conserved_quantities_match_reference(prog, ref):
+    ref.ref_compute_conserved.argtypes = [_D] * 8 + [_PD]
+    rng = np.random.default_rng(33)
+    rows = []
+    for _ in range(30):
+        rows.append([rng.uniform(3, 50), rng.uniform(0.2, 2.9), rng.uniform(0.5, 2.0), rng.normal(),
+                     rng.normal() * 0.1, rng.normal() * 0.05, 1.0, rng.uniform(0, 0.99)])
+    out = prog("conserved", rows)[:-1]
+    for row, o in zip(rows, out):
+        want = np.zeros(4)
+        ref.ref_compute_conserved(*row, P(want))
+        np.testing.assert_allclose(o, want, rtol=1e-12, atol=1e-12)
