@@ -12,10 +12,11 @@ One frame set (any number of cameras at r_obs = 100 M) goes through:
   3. bh_trace_pixel_list_compact + bh_surrogate_audit: the audit sample
      traced into a compact buffer and compared with the surrogate records.
 All steps are asynchronous; list lengths stay on the device.  With
-overlap_audit, the network pass leaves the solve to bh_trace_disk_mino_refine
-so that the audit sample (which needs only the network's audit list) is
-traced on a second stream beside the solve and the fallback, and joined
-back before the comparison.
+overlap_audit (the default), the network pass leaves the solve to
+bh_trace_disk_mino_refine, and after the solve the audit sample is traced
+on a second stream beside the integrator fallback: two short persistent
+launches of the same kernel, each filling the other's tail.  The stream is
+joined back before the comparison.
 """
 from __future__ import annotations
 
@@ -39,9 +40,9 @@ class RenderConfig:
     ambiguity: float = 0.3
     audit_every: int = 100      # 1 % audit
     refine: bool = True         # exact Mino-time crossing solve after the network
-    # trace the audit sample on a second stream beside the solve; off by default:
-    # on c5 the concurrent FP64 kernels contend (63.5-64.1 ms against 61.6-62.5 ms serial)
-    overlap_audit: bool = False
+    # trace the audit sample on a second stream beside the integrator fallback (c5: 56.4 ms
+    # against 57.0 - 57.1 ms in sequence; beside the exact solve instead it was slower)
+    overlap_audit: bool = True
     intensity_floor_rel: float = 1e-6  # audit compares pixels with I_ref > floor x max I
 
 
@@ -130,13 +131,13 @@ class SurrogateRenderer:
             torch = self.torch
             main = torch.cuda.current_stream()
             self._network()
-            listed = torch.cuda.Event()
-            listed.record(main)
-            self.side.wait_event(listed)
+            self.refine()
+            solved = torch.cuda.Event()
+            solved.record(main)
+            self.side.wait_event(solved)
             self.audit_trace(stream=self.side)
             traced = torch.cuda.Event()
             traced.record(self.side)
-            self.refine()
             self.fallback()
             main.wait_event(traced)
             self.audit_compare()

@@ -92,10 +92,11 @@ def test_refined_surrogate_render_matches_oracle():
 
 
 def test_overlapped_audit_pipeline_equals_serial_pipeline():
-    """The pipeline with the audit sample traced on a second stream (network
-    pass with refine = 2, bh_trace_disk_mino_refine, join) gives the records,
-    list lengths and audit statistics of the one-stream pipeline, bit for
-    bit, over repeated runs."""
+    """The pipeline with the audit sample traced on a second stream beside
+    the integrator fallback (network pass with refine = 2,
+    bh_trace_disk_mino_refine, join) gives the records, list lengths and
+    audit statistics of the one-stream pipeline, bit for bit, over repeated
+    runs."""
     import torch
     from blackhole_amd import surrogate_train as ST
     w = workloads.c5(96, spins=(0.3, 0.998), inclinations=(30.0, 80.0))
