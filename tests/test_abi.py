@@ -106,6 +106,27 @@ def test_null_arguments_rejected():
                                       C.c_void_p(1), None) == -1
 
 
+def test_mino_entries_validation_without_gpu():
+    """bh_trace_disk_mino and bh_trace_disk_mino_refine reject bad arguments
+    on the host, naming themselves."""
+    import ctypes as C
+    from blackhole_amd import abi, tracer
+    from blackhole_amd._lib import lib
+    L = lib()
+    cams = (abi.BH_Camera * 1)(tracer.camera(0.5, 60.0, 100.0, 8, 8, 15.0))
+    o = tracer.default_options(disk_enabled=1, volume_enabled=0, disk_r_out=20.0)
+    for name in ("bh_trace_disk_mino", "bh_trace_disk_mino_refine"):
+        fn = getattr(L, name)
+        p1 = C.c_void_p(1)
+        assert fn(cams, 1, C.byref(o), None, 256, p1, p1, p1, 16, None) == -1      # no LUTs
+        assert name.encode() in L.bh_last_error()
+        assert fn(cams, 1, C.byref(o), p1, 256, p1, p1, p1, 0, None) == -1         # no list capacity
+        assert b"fallback_cap" in L.bh_last_error()
+        ov = tracer.default_options(disk_enabled=1, volume_enabled=1, disk_r_out=20.0)
+        assert fn(cams, 1, C.byref(ov), p1, 256, p1, p1, p1, 16, None) == -1       # volumes not supported
+        assert b"thin disk only" in L.bh_last_error()
+
+
 def test_polarised_entry_validation_without_gpu():
     """bh_trace_image_plane_pol rejects its bad arguments on the host."""
     import ctypes as C
