@@ -412,43 +412,53 @@ MINO_HD RaySolution solve_ray(double a, double xi, double eta, double r_o, doubl
   RaySolution out{{0.0, 0.0, 0.0}, 0.0, 0, -1, kHorizon, 0, 0};
   Path p;
   if (!make_path(a, xi, eta, r_o, r_stop, r_escape, &p)) return out;
-  const int end_cls = p.turning ? kEscaped : kHorizon;
   const PolarTimes pt = polar_times(a, xi, eta, u_o, u_dir);
+  out.cls = p.turning ? kEscaped : kHorizon;
+  if (!pt.ok) {  // never reaches the equator: the path's end decides the class
+    out.ok = 1;
+    return out;
+  }
+  // Mino time to the turning point (turning paths) or to the stopping radius
+  // (plunging); a turning path's outbound leg is integrated only when a
+  // crossing lies beyond the turning point, so rays that hit the disk on the
+  // way in never need it.  T_total < 0: not integrated yet.
   int ok = 1;
-  double T4 = 0.0, T_total;
+  double T4 = 0.0, T_total = -1.0;
   if (p.turning) {
     T4 = integrate(p, 0.0, p.s_o, tol, &ok);
-    if (!ok) return out;
-    T_total = T4 + integrate(p, p.s_o, p.w_end, tol, &ok);
   } else {
     T_total = integrate(p, 0.0, p.w_end, tol, &ok);
   }
-  if (!ok) return out;
-  out.cls = end_cls;
-  if (!pt.ok) {
-    out.ok = 1;
+  if (!ok) {
+    out.cls = kHorizon;
     return out;
   }
   double w_prev = 0.0, T_prev = 0.0;
   for (int m = 0; m < kMaxCrossings; ++m) {
     const double G = pt.g0 + m * pt.dg;
-    if (!(G < T_total)) {
-      out.ok = 1;
-      return out;
-    }
-    double lo = w_prev, Tlo = T_prev, hi = p.w_end, Thi = T_total;
+    double lo = w_prev, Tlo = T_prev, hi, Thi;
     bool outbound = false;
-    if (p.turning) {
-      if (G <= T4) {
-        hi = p.s_o;
-        Thi = T4;
-      } else {
+    if (p.turning && G <= T4) {
+      hi = p.s_o;
+      Thi = T4;
+    } else {
+      if (p.turning) {
+        if (T_total < 0.0) {
+          T_total = T4 + integrate(p, p.s_o, p.w_end, tol, &ok);
+          if (!ok) return out;
+        }
         outbound = true;
         if (lo < p.s_o) {
           lo = p.s_o;
           Tlo = T4;
         }
       }
+      if (!(G < T_total)) {  // the path ends (escape or capture) before this crossing
+        out.ok = 1;
+        return out;
+      }
+      hi = p.w_end;
+      Thi = T_total;
     }
     double w0 = -1.0;
     if (m < n_guess && guess[m] > 0.0) w0 = w_of_r(p, guess[m], outbound);
