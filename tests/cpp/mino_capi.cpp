@@ -1,11 +1,18 @@
 // C entry points over the Mino-time crossing solver (blackhole_amd/csrc/mino.hpp)
 // for the CPU tests: the elliptic integral, the radial roots, the polar
 // times and the per-ray solution.
+#include <cmath>
+
 #include "../../blackhole_amd/csrc/mino.hpp"
 
 extern "C" {
 
 double mino_carlson_rf(double x, double y, double z) { return mino::carlson_rf(x, y, z); }
+
+// The Gauss-Kronrod 3-7 rule on x^d over [a, b]; err = |K7 - G3|.
+double mino_gk7_monomial(int d, double a, double b, double* err) {
+  return mino::gk7_rule([d](double x) { return std::pow(x, d); }, a, b, err);
+}
 
 int mino_radial_roots(double A, double B, double C, double* r34) {
   const mino::RadialRoots rr = mino::radial_roots(A, B, C);

@@ -5,6 +5,9 @@ compiled by g++ from the same header the GPU kernel includes:
     negative parameters included (the polar motion's parameter is <= 0);
   * the largest radial root against numpy.roots of the same quartic;
   * the polar Mino times against a direct quadrature of du / sqrt(U);
+  * the Gauss-Kronrod 3-7 rule of the Newton steps: exact on monomials up
+    to degree 11 and not at 12, its embedded 3-point Gauss rule exact up to
+    degree 5 and not at 6 (a wrong node or weight breaks one of these);
   * whole rays against the oracle's step-integrated records on the 16 frames
     of the surrogate comparison (spins 0/0.5/0.9/0.998 x 17/40/60/85 deg) at
     low resolution: identical hit classes, crossing radii within 1e-6.
@@ -35,6 +38,8 @@ def mino(tmp_path_factory):
     L = C.CDLL(str(so))
     L.mino_carlson_rf.restype = _D
     L.mino_carlson_rf.argtypes = [_D] * 3
+    L.mino_gk7_monomial.restype = _D
+    L.mino_gk7_monomial.argtypes = [C.c_int, _D, _D, C.POINTER(_D)]
     L.mino_radial_roots.argtypes = [_D] * 3 + [C.POINTER(_D)]
     L.mino_polar_times.argtypes = [_D] * 5 + [C.POINTER(_D)]
     L.mino_solve_pixel.argtypes = [_D] * 9 + [C.POINTER(_D), C.c_int, _D, C.POINTER(_D), C.POINTER(C.c_int)]
@@ -53,6 +58,20 @@ def test_carlson_rf_is_the_incomplete_elliptic_integral(mino):
     # complete integral: R_F(0, 1 - m, 1) = K(m)
     for m in (-40.0, -1.0, 0.0, 0.5):
         assert abs(mino.mino_carlson_rf(0.0, 1.0 - m, 1.0) - special.ellipk(m)) <= 5e-15 * special.ellipk(m)
+
+
+def test_gauss_kronrod_3_7_degrees(mino):
+    err = _D()
+    for a, b in ((-1.0, 1.0), (0.3, 2.1), (-2.5, -0.4)):
+        for d in range(14):
+            exact = (b ** (d + 1) - a ** (d + 1)) / (d + 1)
+            v = mino.mino_gk7_monomial(d, a, b, C.byref(err))
+            scale = max(abs(a), abs(b)) ** d * (b - a)
+            k_exact = abs(v - exact) <= 1e-14 * scale
+            # the Gauss estimate is v -/+ err: exact iff err vanishes with K7 exact
+            g_exact = k_exact and err.value <= 1e-14 * scale
+            assert k_exact == (d <= 11 or (d % 2 == 1 and a == -b)), (a, b, d, v, exact)
+            assert g_exact == (d <= 5 or (d % 2 == 1 and a == -b)), (a, b, d, err.value)
 
 
 def test_largest_radial_root_matches_numpy(mino):
