@@ -36,9 +36,11 @@
 // integrator (fallback) when its confidence output is low, when it lies
 // within ring_band of the critical curve (per-camera polar table), when a
 // predicted crossing lies within edge_tol of the disk's inner or outer edge,
-// or when the crossing-count / capture outputs are ambiguous.  A
-// deterministic 1-in-audit_every sample of the other rays is listed for the
-// audit.  Appends use one atomic per wave (ballot + popcount).
+// or when the crossing-count / capture outputs are ambiguous (with the
+// Mino-time solve to follow, only the ring band; with the band solved too,
+// none: band rays are written with no crossings for the solver to start
+// afresh).  A deterministic 1-in-audit_every sample of the other rays is
+// listed for the audit.  Appends use one atomic per wave (ballot + popcount).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -98,7 +100,8 @@ struct InferParams {
   double disk_r_out, nu, T_max, r_escape;
   int n_cams, lut_n, n_hidden, audit_every;  // weight layers 1..n_hidden are 256 x 256
   float ring_band, edge_tol, conf_max, amb;
-  int refine;  // the Mino-time solver follows: only the ring band falls back here
+  int refine;      // the Mino-time solver follows: only the ring band falls back here
+  int band_solve;  // ... and solves the ring band too (no crossings written, nothing listed)
 };
 
 __device__ __forceinline__ uint32_t hash32(uint64_t x) {
@@ -348,6 +351,7 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
     const float cap = O[BH_SUR_CAPTURE];
     const float conf = O[BH_SUR_CONF];
     const int nc = static_cast<int>(fminf(fmaxf(rintf(ncf), 0.0f), 3.0f));
+    bool band = false;
     fb = !P.refine && (!(conf <= P.conf_max) || fabsf(ncf - rintf(ncf)) > P.amb);
     // critical-curve band: radial distance from the curve's [lo, hi] range in
     // the pixel's polar bin, against the band scaled by the bin's obliquity
@@ -360,7 +364,8 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
       const float rho = sqrtf(da * da + db * db);
       const float lo = rg[2 + bin], hi = rg[2 + BH_SUR_RING_BINS + bin], obl = rg[2 + 2 * BH_SUR_RING_BINS + bin];
       const float d = fmaxf(fmaxf(lo - rho, rho - hi), 0.0f);
-      if (d < P.ring_band * obl) fb = true;
+      band = d < P.ring_band * obl;
+      if (band && !P.band_solve) fb = true;
     }
     double cr0 = 0.0, cr1 = 0.0, cr2 = 0.0, cg0 = 0.0, cg1 = 0.0, cg2 = 0.0, I = 0.0;
     int hit = -1;
@@ -385,7 +390,7 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
     }
     if (!P.refine && hit < 0 && fabsf(cap - rintf(cap)) > P.amb) fb = true;
     const int cls = hit >= 0 ? BH_HIT_DISK : (cap > 0.5f ? BH_HIT_HORIZON : BH_HIT_ESCAPED);
-    const int ncr = hit >= 0 ? hit + 1 : nc;
+    const int ncr = (band && P.band_solve) ? 0 : (hit >= 0 ? hit + 1 : nc);
     double r_exit = 0.0, u_exit = 0.0, phi_exit = 0.0;
     if (cls == BH_HIT_DISK) {
       r_exit = hit == 0 ? cr0 : (hit == 1 ? cr1 : cr2);
@@ -638,7 +643,7 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
     return fail("bh_surrogate_render: NULL argument");
   if (fallback_cap <= 0 || audit_cap <= 0) return fail("bh_surrogate_render: list capacities must be > 0");
   if (!(rp->ring_band >= 0.0f && rp->edge_tol >= 0.0f && rp->ambiguity >= 0.0f) || rp->audit_every < 0 ||
-      rp->refine < 0 || rp->refine > 2)
+      rp->refine < 0 || rp->refine > 3)
     return fail("bh_surrogate_render: invalid render parameters");
   const SurView v = surrogate_view(s);
   if (v.width != kW || v.L < 2 || v.L > kMaxLayers)
@@ -715,6 +720,7 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
   P.conf_max = rp->conf_max;
   P.amb = rp->ambiguity;
   P.refine = rp->refine != 0;
+  P.band_solve = rp->refine == 3;
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P.fb_count, P.audit_count);
   const long long nblk = (total + kTile - 1) / kTile;
   if (nblk > 0x7fffffffll) return fail("bh_surrogate_render: too many pixels");

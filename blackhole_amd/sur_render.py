@@ -3,10 +3,11 @@
 One frame set (any number of cameras at r_obs = 100 M) goes through:
   1. bh_surrogate_render: every pixel through the fused BF16-MFMA network
      and decode (records, fallback id list, audit id list); with refine set
-     (the default) each ray outside the photon-ring band then has its
-     crossings and hit class solved exactly from the Mino-time integrals of
-     motion, starting from the network's radii, and the rays that solver
-     cannot resolve join the fallback list;
+     (the default) each ray then has its crossings and hit class solved
+     exactly from the Mino-time integrals of motion, starting from the
+     network's radii; the photon-ring band goes to the integrator (or, with
+     solve_band, to the same solver from its own cubic Hermite start), and
+     the rays that solver cannot resolve join the fallback list;
   2. bh_trace_pixel_list: the fallback rays traced by the plain integrator
      into the same image (records overwrite the surrogate's);
   3. bh_trace_pixel_list_compact + bh_surrogate_audit: the audit sample
@@ -40,6 +41,11 @@ class RenderConfig:
     ambiguity: float = 0.3
     audit_every: int = 100      # 1 % audit
     refine: bool = True         # exact Mino-time crossing solve after the network
+    # the solve also takes the ring band (from its own starting points) instead of
+    # listing it for the integrator: c5 then has no integrator rays, but near the
+    # critical curve the quadrature needs many panels and those waves set the solve's
+    # tail (refinement 10.4 against 7.3 ms; pipeline 55.8 against 54.8 ms)
+    solve_band: bool = False
     # trace the audit sample on a second stream beside the integrator fallback (c5: 56.4 ms
     # against 57.0 - 57.1 ms in sequence; beside the exact solve instead it was slower)
     overlap_audit: bool = True
@@ -78,8 +84,9 @@ class SurrogateRenderer:
         self.ring = torch.from_numpy(ring_tables(scene.cams).reshape(-1)).to(dev)
         split = cfg.refine and cfg.overlap_audit and cfg.audit_every > 0
         self.side = torch.cuda.Stream(device=dev) if split else None  # default priority
+        mode = 0 if not cfg.refine else (3 if cfg.solve_band else (2 if split else 1))
         self.params = S.RenderParams(cfg.ring_band, cfg.edge_tol, cfg.conf_max, cfg.ambiguity,
-                                     cfg.audit_every, 2 if split else int(cfg.refine))
+                                     cfg.audit_every, mode)
         self.floor_I = 0.0
 
     def _stream(self):
@@ -88,11 +95,11 @@ class SurrogateRenderer:
     def render(self) -> None:
         """Surrogate pass (records + lists), with the exact solve when refine is set."""
         self._network()
-        if self.params.refine == 2:
+        if self.params.refine >= 2:
             self.refine()
 
     def refine(self) -> None:
-        """The Mino-time solve of the network's records (refine = 2 pass)."""
+        """The Mino-time solve of the network's records (refine = 2 or 3 pass)."""
         cam_arr = (abi.BH_Camera * len(self.sc.cams))(*self.sc.cams)
         p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
         check(lib().bh_trace_disk_mino_refine(
@@ -127,7 +134,7 @@ class SurrogateRenderer:
               "bh_surrogate_audit")
 
     def run(self, audit: bool = True) -> None:
-        if self.params.refine == 2 and audit:
+        if self.side is not None and audit:
             torch = self.torch
             main = torch.cuda.current_stream()
             self._network()

@@ -119,7 +119,12 @@ typedef struct BH_SurrogateRenderParams {
                              ring band and the solver's own failures fall back.
                              2: the same decode, the solve left to the caller
                              (bh_trace_disk_mino_refine, e.g. on the render's
-                             stream while another stream traces the audit list) */
+                             stream while another stream traces the audit list).
+                             3: as 2, and the ring band is solved too: its rays are
+                             not listed, their records carry no crossings (the
+                             solve starts them from its own cubic Hermite model),
+                             the audit sample is drawn from every ray, and only the
+                             solver's failures reach the fallback list */
 } BH_SurrogateRenderParams;
 
 /* Render every pixel of the cameras with the fused surrogate kernel

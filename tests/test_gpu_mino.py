@@ -91,21 +91,23 @@ def test_refined_surrogate_render_matches_oracle():
     assert it_guess <= 4.0 and it_fresh <= 4.0
 
 
-def test_overlapped_audit_pipeline_equals_serial_pipeline():
+@pytest.mark.parametrize("solve_band,modes", [(False, (2, 1)), (True, (3, 3))])
+def test_overlapped_audit_pipeline_equals_serial_pipeline(solve_band, modes):
     """The pipeline with the audit sample traced on a second stream beside
-    the integrator fallback (network pass with refine = 2,
+    the integrator fallback (network pass with refine = 2 or 3,
     bh_trace_disk_mino_refine, join) gives the records, list lengths and
     audit statistics of the one-stream pipeline, bit for bit, over repeated
     runs."""
     import torch
     from blackhole_amd import surrogate_train as ST
     w = workloads.c5(96, spins=(0.3, 0.998), inclinations=(30.0, 80.0))
-    print("[overlap] 4 x 96^2 rays", flush=True)
+    print("[overlap] 4 x 96^2 rays, solve_band", solve_band, flush=True)
     sc = tracer.upload_scene(w["cams"], w["opts"], w["luts"])
     sur = ST.load(WEIGHTS)
-    A = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(audit_every=7, overlap_audit=True))
-    B = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(audit_every=7, overlap_audit=False))
-    assert A.side is not None and A.params.refine == 2 and B.side is None and B.params.refine == 1
+    cfg = dict(audit_every=7, solve_band=solve_band)
+    A = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(overlap_audit=True, **cfg))
+    B = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(overlap_audit=False, **cfg))
+    assert A.side is not None and B.side is None and (A.params.refine, B.params.refine) == modes
     B.run()
     torch.cuda.synchronize()
     want = B.out.cpu().numpy()
@@ -157,3 +159,45 @@ def test_trained_network_improves_on_its_prior_and_on_an_untrained_network():
     errs["prior"] = float(np.concatenate(prior).mean())
     print("[gain] mean |d ln r_1|:", errs, flush=True)
     assert errs["trained"] < 0.5 * errs["prior"] and errs["trained"] < 0.5 * errs["untrained"]
+
+
+def test_band_solved_by_the_mino_solver_matches_the_integrator():
+    """With the ring band solved (refine = 3) instead of traced: records
+    outside the band bit for bit those of the band-to-integrator pipeline;
+    band rays refined from the solver's own start, at the integrator's class
+    and crossings (the oracle's, checked as for every other solved ray);
+    fallback only for the solver's failures; the audit sample drawn from
+    every ray, band included."""
+    import torch
+    from blackhole_amd import surrogate_train as ST
+    w = workloads.c5(64, spins=(0.5, 0.998), inclinations=(30.0, 80.0))
+    print("[band] 4 x 64^2 rays", flush=True)
+    sc = tracer.upload_scene(w["cams"], w["opts"], w["luts"])
+    sur = ST.load(WEIGHTS)
+    A = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(audit_every=7, solve_band=True))
+    B = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(audit_every=7, solve_band=False))
+    A.run()
+    B.run()
+    torch.cuda.synchronize()
+    fa, aa = A.counts_host()
+    fb, ab = B.counts_host()
+    ga, gb = tracer.records_to_numpy(A.out), tracer.records_to_numpy(B.out)
+    band = np.zeros(len(ga), bool)
+    band[B.fb_ids[:fb].cpu().numpy()] = True
+    print("[band] band rays", int(band.sum()), "fallback with band solved", fa, flush=True)
+    assert band.sum() > 100 and fa < fb
+    assert np.array_equal(A.out.cpu().numpy()[~band], B.out.cpu().numpy()[~band])
+    a_fb = np.zeros(len(ga), bool)
+    a_fb[A.fb_ids[:fa].cpu().numpy()] = True
+    assert not (a_fb & ~band).any()                  # outside the band the lists agree
+    solved = band & ~a_fb
+    assert ((ga["flags"][solved] & abi.FLAG_REFINED) != 0).all()
+    ref = OL.trace(w["cams"], w["opts"], luts=w["luts"], lut_n=w["luts"].shape[1])
+    assert np.array_equal(ga["hit_class"], ref["hit_class"])
+    _check_disk_records(ga, ref, solved)
+    assert np.array_equal(ga["hit_class"][band], gb["hit_class"][band])
+    au = A.audit_ids[:aa].cpu().numpy()
+    assert band[au].any() and len(np.unique(au)) == aa
+    assert abs(aa - len(ga) / 7) < 0.1 * len(ga) / 7
+    st = A.audit_stats()
+    assert st["class_diff"] == 0 and st["mean_rel"] <= 1e-6
