@@ -292,8 +292,10 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
     }
   }
   AFrag pa[kPrefetch];  // the next layer's first K-steps, in flight across the epilogue
+  if (P.n_hidden > 0) {
 #pragma unroll
-  for (int d = 0; d < kPrefetch; ++d) pa[d] = load_afrag(P.wsw, o0, lane, d);
+    for (int d = 0; d < kPrefetch; ++d) pa[d] = load_afrag(P.wsw, o0, lane, d);
+  }
   store_act(acc, H, o0, l32, lh);
 
   // ---- hidden layers 1..n_hidden: 256 x 256 ----------------------------------------

@@ -86,17 +86,20 @@ def test_surrogate_forward_matches_checker_and_training_reduces_loss():
     assert last < 0.05 * first
 
 
-def test_fused_render_matches_layerwise_forward_checker_and_decode():
+@pytest.mark.parametrize("depth", [6, 1])
+def test_fused_render_matches_layerwise_forward_checker_and_decode(depth):
     """The fused render kernel's raw network outputs (confidence output and the
     first ln r output, recovered from the records) against the layer-by-layer
     forward and the FP64 checker; the decode (g, intensity, flags, lists)
-    against the oracle's disk formulas on the host."""
+    against the oracle's disk formulas on the host.  Depth 1 is the network
+    with no hidden 256 x 256 layer (layer 0 straight into the output layer)."""
     import torch
     import mlp_oracle
     import oracle_lib as OL
     from blackhole_amd import abi, analysis, sur_render, tracer
     from blackhole_amd import surrogate as S
-    sur = S.Surrogate(width=256, depth=6, max_batch=4096, seed=21)
+    print("fused render, depth", depth, flush=True)
+    sur = S.Surrogate(width=256, depth=depth, max_batch=4096, seed=21)
     cams = [tracer.camera(0.3, 60.0, 100.0, 48, 40, 20.0), tracer.camera(0.9, 17.0, 100.0, 32, 32, 20.0)]
     cams[0].record_offset = 0
     cams[1].record_offset = 48 * 40 + 100  # a gap of 100 records
