@@ -34,6 +34,12 @@ Other workloads (one JSON line each, same contract):
   --workload c2  the volumetric FP64 DP45 tracer on the c2 frame (1024^2,
                  a = 0.94, 17 deg, torus 256 x 128 x 64, 230 GHz).
 
+Every roofline is priced against MI355X's published peak (``peak``, ``frac``);
+after the workload the library's micro-kernels measure the device's
+attainable FP64 / FP32 FMA, BF16 MFMA and copy rates (``measured_peaks``),
+and each roofline also carries ``peak_measured`` and ``frac_measured``
+(``--no-peaks`` skips them).
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c4|c3|c2]
 """
 from __future__ import annotations
@@ -518,6 +524,39 @@ def bench_c2(args):
     return res
 
 
+def measured_peaks() -> dict:
+    """The device's attainable peaks from the library's micro-kernels
+    (bh_measure_peaks: FMA chains, MFMA chains, a 2 GiB copy)."""
+    import ctypes as C
+    from blackhole_amd._lib import check, lib
+    out = (C.c_double * 4)()
+    check(lib().bh_measure_peaks(out, None), "bh_measure_peaks")
+    return {"fp64_fma_tflops": out[0], "fp32_fma_scalar_tflops": out[1], "bf16_mfma_tflops": out[2],
+            "hbm_copy_gbs": out[3],
+            "note": "micro-kernels on this device: 8 FMA chains per lane (FP32 without packed "
+                    "instructions), 4 v_mfma_f32_32x32x16_bf16 chains per wave, 2 GiB non-temporal copy (read + write)"}
+
+
+def attach_measured_peaks(res: dict, pk: dict) -> None:
+    """Adds peak_measured / frac_measured beside every roofline's published peak."""
+    table = {FP64_VECTOR_PEAK_TFLOPS: pk["fp64_fma_tflops"], 157.3: pk["fp32_fma_scalar_tflops"],
+             BF16_DENSE_PEAK_TFLOPS: pk["bf16_mfma_tflops"]}
+
+    def walk(d):
+        if isinstance(d, dict):
+            if "achieved" in d and d.get("peak") in table:
+                d["peak_measured"] = table[d["peak"]]
+                d["frac_measured"] = d["achieved"] / d["peak_measured"]
+            for v in d.values():
+                walk(v)
+        elif isinstance(d, list):
+            for v in d:
+                walk(v)
+
+    walk(res)
+    res["measured_peaks"] = pk
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -529,6 +568,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=8_000_000, help="c5: oracle pixels for the CPU baseline")
     ap.add_argument("--cpu-stride", type=int, default=1, help="c2: oracle pixel stride")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64", help="c2 only")
+    ap.add_argument("--no-peaks", action="store_true", help="skip the peak-measuring micro-kernels")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device copy (profiler counter calibration)")
     args = ap.parse_args()
@@ -541,6 +581,8 @@ def main() -> int:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
     res = {"c5": bench_c5, "c4": bench_c4, "c3": bench_c3, "c2": bench_c2}[args.workload](args)
+    if not args.no_peaks:
+        attach_measured_peaks(res, measured_peaks())
     print(json.dumps(res), flush=True)
     return 0
 

@@ -95,6 +95,8 @@ def lib():
     L.bh_last_kernel_ms.restype = C.c_float
     L.bh_trace_disk_mino.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.BH_TraceOptions), C.c_void_p, C.c_int,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+    L.bh_measure_peaks.restype = C.c_int
+    L.bh_measure_peaks.argtypes = [C.POINTER(_D), C.c_void_p]
     L.bh_trace_disk_mino.restype = C.c_int
     L.bh_trace_disk_mino_refine.argtypes = L.bh_trace_disk_mino.argtypes
     L.bh_trace_disk_mino_refine.restype = C.c_int
