@@ -68,7 +68,7 @@ WEIGHTS = os.path.join(ROOT, "blackhole_amd", "weights", "surrogate_c5.npz")
 
 
 def cpu_baseline(w, gpu_rec_np_fn, ids, label: str):
-    """Time the FP64 CPU oracle (the checker) on the listed pixels and check
+    """Time the FP64 CPU oracle on the listed pixels and check
     the GPU records at those pixels against it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib

@@ -1,7 +1,7 @@
 // flopcount.cpp - algorithmic FLOP per unit of work of the tracer, counted by
 // instantiating the oracle's templates with a counting scalar.
 //
-// TEST / MEASUREMENT INFRASTRUCTURE ONLY (the checker): prints one JSON object
+// TEST / MEASUREMENT INFRASTRUCTURE ONLY: prints one JSON object
 // with the FLOP per RHS evaluation, per RK4 step, per DP45 attempt, per
 // equatorial crossing, per disk hit, per volume sample and per ray setup.
 // Counting rules: + - * / sqrt = 1 each, FMA = 2, negation / fabs /

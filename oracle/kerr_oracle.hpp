@@ -1,6 +1,6 @@
 // kerr_oracle.hpp - FP64 CPU oracle of the per-pixel Kerr null-geodesic tracer.
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker: it is compiled into
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle: it is compiled into
 // oracle/liboracle.so, which only tests/, __graft_entry__.smoke() and the
 // cpu_baseline leg of bench.py load.  The product (blackhole_amd/) never
 // includes, links or calls it.

@@ -1,12 +1,12 @@
 """FP64 numpy restatement of the surrogate network and its loss gradient.
 
-TEST INFRASTRUCTURE ONLY (the checker of the GPU training kernels): only
+TEST INFRASTRUCTURE ONLY (the FP64 reference of the GPU training kernels): only
 tests/ import it.  Same architecture and loss as the GPU path
 (include/bhk_surrogate.h): features padded to 32 -> depth x width SiLU ->
 16 outputs; loss = sum_rows [sum_c w (y - t)^2 + 0.01 (y_conf - log10(err + 1e-12))^2] / n
 with err = sum_c w (y - t)^2 (the confidence column excluded, its target
-detached).  The GPU keeps activations and incoming gradients in BF16; the
-checker evaluates in FP64 on the same BF16-rounded weights and inputs.
+detached).  The GPU keeps activations and incoming gradients in BF16; this
+restatement evaluates in FP64 on the same BF16-rounded weights and inputs.
 """
 from __future__ import annotations
 

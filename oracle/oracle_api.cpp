@@ -1,7 +1,8 @@
 // oracle_api.cpp - extern "C" API of the FP64 CPU oracle (liboracle.so).
 //
 // TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
-// bench.py's cpu_baseline leg, as the checker.  The product never loads it.
+// bench.py's cpu_baseline leg, as the reference of their checks.  The product
+// never loads it.
 //
 // The per-ray loop restates KerrRaytracer::trace (src/physics/raytracer.h:
 // 324-378) -- capture / escape / step-budget termination -- with the

@@ -1,7 +1,7 @@
 """ctypes loader for the FP64 CPU oracle (oracle/liboracle.so).
 
 Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg use this module, as the checker.
+cpu_baseline leg use this module, as the reference of their checks.
 """
 from __future__ import annotations
 

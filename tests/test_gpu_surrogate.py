@@ -1,5 +1,5 @@
 """Surrogate MLP kernels on the GPU: MFMA GEMM against numpy, training
-gradients against the FP64 checker (oracle/mlp_oracle.py), and a short
+gradients against the FP64 oracle (oracle/mlp_oracle.py), and a short
 training run that must reduce the loss."""
 from __future__ import annotations
 
@@ -29,7 +29,7 @@ def test_gemm_nt_mfma_layout_and_values():
         assert err <= 1e-4 * (ref.abs().max().item() + 1)
 
 
-def test_surrogate_gradients_match_fp64_checker():
+def test_surrogate_gradients_match_fp64_oracle():
     import torch
     import mlp_oracle
     from blackhole_amd import surrogate as S
@@ -59,7 +59,7 @@ def test_surrogate_gradients_match_fp64_checker():
         off += k
 
 
-def test_surrogate_forward_matches_checker_and_training_reduces_loss():
+def test_surrogate_forward_matches_oracle_and_training_reduces_loss():
     import torch
     import mlp_oracle
     from blackhole_amd import surrogate as S
@@ -87,10 +87,10 @@ def test_surrogate_forward_matches_checker_and_training_reduces_loss():
 
 
 @pytest.mark.parametrize("depth", [6, 1])
-def test_fused_render_matches_layerwise_forward_checker_and_decode(depth):
+def test_fused_render_matches_layerwise_forward_oracle_and_decode(depth):
     """The fused render kernel's raw network outputs (confidence output and the
     first ln r output, recovered from the records) against the layer-by-layer
-    forward and the FP64 checker; the decode (g, intensity, flags, lists)
+    forward and the FP64 oracle; the decode (g, intensity, flags, lists)
     against the oracle's disk formulas on the host.  Depth 1 is the network
     with no hidden 256 x 256 layer (layer 0 straight into the output layer)."""
     import torch

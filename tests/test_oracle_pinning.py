@@ -1,4 +1,4 @@
-"""Pins the FP64 CPU oracle before it is trusted as the checker.
+"""Pins the FP64 CPU oracle before the GPU results are checked against it.
 
 Sources, in order of preference:
   * golden fixtures of the reference (tests/golden/*.csv, copied from the
