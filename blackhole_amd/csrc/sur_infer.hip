@@ -375,7 +375,9 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
     for (int c = 0; c < 3; ++c) {
       if (c < nc && hit < 0) {
         const double r = exp(static_cast<double>(O[2 * c] + lnbase));
-        const double g = (r >= k.r_isco) ? disk_g(r, a, xi) : 0.0;
+        // with the solve to follow, g and the intensity are written by the solve
+        // (or the integrator) for every ray, so only the radii are decoded
+        const double g = (!P.refine && r >= k.r_isco) ? disk_g(r, a, xi) : 0.0;
         if (c == 0) { cr0 = r; cg0 = g; }
         if (c == 1) { cr1 = r; cg1 = g; }
         if (c == 2) { cr2 = r; cg2 = g; }
@@ -384,9 +386,11 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
           fb = true;
         if (r >= k.r_isco && r <= P.disk_r_out) {
           hit = c;
-          const double* lut = P.luts + static_cast<long long>(k.lut_index) * P.lut_n;
-          const double F = lut_linear(lut, P.lut_n, k.r_isco, P.disk_r_out, r);
-          I = disk_intensity(F, g, P.nu, P.T_max);
+          if (!P.refine) {
+            const double* lut = P.luts + static_cast<long long>(k.lut_index) * P.lut_n;
+            const double F = lut_linear(lut, P.lut_n, k.r_isco, P.disk_r_out, r);
+            I = disk_intensity(F, g, P.nu, P.T_max);
+          }
         }
       }
     }

@@ -116,7 +116,10 @@ typedef struct BH_SurrogateRenderParams {
                              the Mino-time integrals (bh_trace_disk_mino's solver) with
                              the network's radii as starting points; confidence,
                              count and edge criteria are then not applied, only the
-                             ring band and the solver's own failures fall back.
+                             ring band and the solver's own failures fall back.  The
+                             render then decodes the crossing radii, count and class
+                             only: g and intensity are left 0 for the solve (and the
+                             integrator, for listed rays) to write.
                              2: the same decode, the solve left to the caller
                              (bh_trace_disk_mino_refine, e.g. on the render's
                              stream while another stream traces the audit list).
