@@ -127,6 +127,14 @@ def test_mino_entries_validation_without_gpu():
         assert b"thin disk only" in L.bh_last_error()
 
 
+def test_measure_peaks_validation_without_gpu():
+    """bh_measure_peaks rejects a NULL output before touching the device."""
+    from blackhole_amd._lib import lib
+    L = lib()
+    assert L.bh_measure_peaks(None, None) == -1
+    assert b"bh_measure_peaks" in L.bh_last_error()
+
+
 def test_polarised_entry_validation_without_gpu():
     """bh_trace_image_plane_pol rejects its bad arguments on the host."""
     import ctypes as C
