@@ -1,7 +1,8 @@
 #!/usr/bin/env bash
 # Compiles the reference's own FP64 physics functions, from their sources in
 # /root/reference/src/physics (kerr.cpp + header-only formulas), together with
-# oracle/ref/ref_wrap.cpp into oracle/_ref/libref.so, and the reference's
+# oracle/ref/ref_wrap.cpp and oracle/ref/ref_wrap_disk.cpp into
+# oracle/_ref/libref.so, and the reference's
 # render-ABI header into oracle/_ref/abi_layout (prints BH_LaunchParams
 # offsets).  Test infrastructure only; outputs stay in oracle/_ref/.
 set -euo pipefail
@@ -15,7 +16,8 @@ fi
 mkdir -p "$OUT"
 CXX_REF="${CXX_REF:-/opt/rocm/llvm/bin/clang++}"
 "$CXX_REF" -std=c++23 -O2 -ffp-contract=off -fPIC -shared -w \
-  -I "$REF/src" "$HERE/ref/ref_wrap.cpp" "$REF/src/physics/kerr.cpp" -o "$OUT/libref.so"
+  -I "$REF/src" "$HERE/ref/ref_wrap.cpp" "$HERE/ref/ref_wrap_disk.cpp" "$REF/src/physics/kerr.cpp" \
+  -o "$OUT/libref.so"
 cat > "$OUT/abi_layout.c" <<'CEOF'
 #include <stdio.h>
 #include <stddef.h>

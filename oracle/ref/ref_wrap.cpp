@@ -71,6 +71,11 @@ void ref_rte_step(double* state3, double j, double alpha, double ds) {
 
 double ref_planck(double nu, double T) { return physics::planckFunction(nu, T); }
 
+// rte_integrator.h:465-515: frame transforms of I, j and alpha by g.
+double ref_gr_transform_intensity(double i_emit, double g) { return physics::grTransformIntensity(i_emit, g); }
+double ref_gr_transform_emission(double j_emit, double g) { return physics::grTransformEmission(j_emit, g); }
+double ref_gr_transform_absorption(double a_emit, double g) { return physics::grTransformAbsorption(a_emit, g); }
+
 double ref_synch_j(double nu, double B, double ne, double th) {
   return physics::synchrotronThermalEmissivity(nu, B, ne, th);
 }

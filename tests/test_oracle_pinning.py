@@ -80,6 +80,12 @@ def ref():
     R.ref_synch_alpha.argtypes = [_D] * 4
     R.ref_synch_alpha.restype = _D
     R.ref_kerr_step_mino.argtypes = [_D, _D, _D, _D, _PD, _D]
+    R.ref_disk_flux.argtypes = [_D] * 4
+    R.ref_disk_flux.restype = _D
+    for name in ("ref_gr_transform_intensity", "ref_gr_transform_emission", "ref_gr_transform_absorption"):
+        getattr(R, name).argtypes = [_D, _D]
+        getattr(R, name).restype = _D
+    R.ref_kerr_gcon.argtypes = [_D, _D, _D, _D, _PD]
     return R
 
 
@@ -564,3 +570,219 @@ def test_oracle_against_committed_reference_vectors(L):
     for v in vec["disk_g"]:
         r, phi, a, inc = v["in"]
         assert abs(L.orc_disk_g(r, 0.0, r * math.sin(phi) * math.sin(inc)) - v["out"]) <= 1e-13 * v["out"]
+
+
+# ---------------------------------------------------------------------------
+# Disk intensity, LUT sampling, NT spin term, fluid redshift and the volume
+# frame transforms
+# ---------------------------------------------------------------------------
+
+H_CGS = 2.0 * math.pi * 1.054571817e-27  # Planck's constant, erg s
+KB_CGS = 1.380649e-16
+C_CGS = 2.99792458e10
+
+
+@pytest.mark.parametrize("a", [0.5, 0.9, 0.998])
+def test_disk_lut_spin_term_matches_reference_diskflux(L, ref, a):
+    """thin_disk.h:243-266 diskFlux, Kerr branch: (1 - sqrt(r_in/r))
+    (1 + a/2 sqrt(M/r)) / r^3 up to a constant.  The oracle's table, at the
+    reference's own range [r_isco, 4 r_isco] and at the c1 disk [r_isco, 20],
+    equals the reference's flux at the same radii divided by its largest
+    sample.  A dropped spin factor, a sign error in it or sqrt(r) in place of
+    sqrt(1/r) each move the normalised table by more than 1e-3."""
+    r_in = L.orc_isco(a)
+    for r_out in (4.0 * r_in, 20.0):
+        n = 256
+        got = np.zeros(n)
+        assert L.orc_disk_lut(a, r_in, r_out, n, P(got)) == 0
+        rr = r_in + (np.arange(n) / (n - 1.0)) * (r_out - r_in)
+        theirs = np.array([ref.ref_disk_flux(r, a, r_in, r_out * (1 + 1e-13)) for r in rr])
+        theirs /= theirs.max()
+        np.testing.assert_allclose(got, theirs, rtol=1e-12, atol=1e-15)
+        # the spin factor is visible at this tolerance
+        no_spin = (1 - np.sqrt(r_in / rr)) / rr**3
+        assert np.max(np.abs(no_spin / no_spin.max() - got)) > 1e-3
+
+
+def test_disk_intensity_is_redshifted_planck(L, ref):
+    """disk_intensity(F, g, nu, T_max) = g^3 B_{nu/g}(T_max F^1/4): the
+    reference's planckFunction at the emitted frequency, carried to the
+    observer by its grTransformIntensity (rte_integrator.h:221-230, 465-475).
+    Rayleigh-Jeans and Wien limits from independent constants; zero for
+    F <= 0 or g <= 0."""
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        F = 10 ** rng.uniform(-4, 0)
+        g = rng.uniform(0.05, 1.8)
+        nu = 10 ** rng.uniform(15, 18.5)
+        T_max = 10 ** rng.uniform(5.5, 7.5)
+        T = T_max * F ** 0.25
+        want = ref.ref_gr_transform_intensity(ref.ref_planck(nu / g, T), g)
+        got = L.orc_disk_intensity(F, g, nu, T_max)
+        if want == 0.0:
+            assert got == 0.0
+            continue
+        # exp(x) turns the rounding of x = h nu / (g k T) into a relative
+        # error of about x ulp
+        x = H_CGS * nu / (g * KB_CGS * T)
+        assert abs(got - want) <= 4e-16 * (x + 100.0) * want, (F, g, nu, T_max)
+    # Rayleigh-Jeans: x = h nu / (g k T) << 1  ->  I = 2 nu^2 g k T / c^2 (1 - x/2)
+    F, g, T_max = 0.3, 0.7, 1e7
+    T = T_max * F ** 0.25
+    nu = 1e-6 * g * KB_CGS * T / H_CGS
+    rj = 2 * nu**2 * g * KB_CGS * T / C_CGS**2
+    assert abs(L.orc_disk_intensity(F, g, nu, T_max) / rj - 1.0) <= 2e-6
+    # Wien: x >> 1  ->  I = 2 h nu^3 / c^2 exp(-x)
+    nu = 40.0 * g * KB_CGS * T / H_CGS
+    wien = 2 * H_CGS * nu**3 / C_CGS**2 * math.exp(-40.0)
+    assert abs(L.orc_disk_intensity(F, g, nu, T_max) / wien - 1.0) <= 1e-8
+    for F, g in ((0.0, 1.0), (-1.0, 1.0), (0.5, 0.0), (0.5, -0.2)):
+        assert L.orc_disk_intensity(F, g, 2.4e17, 1e7) == 0.0
+
+
+def test_lut_linear_nodes_linearity_and_clamp(L):
+    """Linear table sampling over [x0, x1]: node values, midpoints as the mean
+    of their nodes, exact on linear data, clamped to the end values outside
+    the range (NaN reads the first node).  n in place of n - 1, or an index
+    off by one, fails the node check."""
+    rng = np.random.default_rng(5)
+    for n in (2, 3, 17, 256):
+        tab = rng.uniform(-3, 5, n)
+        x0, x1 = rng.uniform(-5, 2), rng.uniform(3, 30)
+        dx = (x1 - x0) / (n - 1)
+        for i in range(n):
+            got = L.orc_lut_linear(P(tab), n, x0, x1, x0 + i * dx)
+            assert abs(got - tab[i]) <= 1e-12 * np.max(np.abs(tab)), (n, i)
+        for i in range(n - 1):
+            got = L.orc_lut_linear(P(tab), n, x0, x1, x0 + (i + 0.5) * dx)
+            assert abs(got - 0.5 * (tab[i] + tab[i + 1])) <= 1e-12 * np.max(np.abs(tab))
+        c0, c1 = rng.uniform(-2, 2), rng.uniform(-2, 2)
+        lin = c0 + c1 * np.arange(n)
+        for x in rng.uniform(x0, x1, 50):
+            assert abs(L.orc_lut_linear(P(lin), n, x0, x1, x) - (c0 + c1 * (x - x0) / dx)) <= 1e-12 * (1 + abs(c1) * n)
+        for x in (x0 - 1.0, x0 - 1e6, -math.inf):
+            assert L.orc_lut_linear(P(tab), n, x0, x1, x) == tab[0]
+        for x in (x1, x1 + 0.5, 1e9, math.inf):
+            assert abs(L.orc_lut_linear(P(tab), n, x0, x1, x) - tab[-1]) <= 1e-12 * np.max(np.abs(tab))
+        assert L.orc_lut_linear(P(tab), n, x0, x1, math.nan) == tab[0]
+
+
+def _ell_kepler(r, a):
+    """Specific angular momentum -u_phi/u_t of the prograde circular
+    equatorial orbit (Bardeen, Press & Teukolsky 1972, M = 1)."""
+    sr = math.sqrt(r)
+    return (r * r - 2 * a * sr + a * a) / (r * sr - 2 * sr + a)
+
+
+def _g_from_reference_gcon(ref, r, u, a, xi, ell):
+    """Redshift of gas with u_phi = -ell u_t seen by a photon (E = 1, L = xi),
+    from the reference's inverse Kerr metric (connection.h:127-178):
+    u_t = -1/sqrt(-(g^tt - 2 ell g^tphi + ell^2 g^phiphi)), u^mu = g^mu nu u_nu,
+    g = 1 / (u^t - xi u^phi).  None where the orbit is superluminal."""
+    G = np.zeros(16)
+    ref.ref_kerr_gcon(r, math.acos(u), a, 2.0, P(G))
+    gtt, gtp, gpp = G[0], G[3], G[15]
+    D = gtt - 2 * ell * gtp + ell * ell * gpp
+    if D >= 0:
+        return None
+    ut_lo = -1.0 / math.sqrt(-D)
+    up_lo = -ell * ut_lo
+    ut_hi = gtt * ut_lo + gtp * up_lo
+    up_hi = gtp * ut_lo + gpp * up_lo
+    return 1.0 / (ut_hi - xi * up_hi)
+
+
+def test_fluid_g_kepler_identity_zamo_and_reference_metric(L, ref):
+    """fluid_g(r, u, a, xi, ell), the redshift of the torus gas:
+      * on the equator with the Keplerian ell it is the thin-disk g
+        (disk_g, pinned against kerrDiskGFactor);
+      * at ell = 0 (zero angular momentum observers) it is the closed form
+        alpha / (1 - omega xi), alpha^2 = Delta Sigma / A, omega = 2 a r / A;
+      * everywhere it equals the redshift built from the reference's
+        contravariant metric kerrGcon.
+    A wrong sign on g_tphi, or sin^2 dropped from g_phiphi, fails all three."""
+    rng = np.random.default_rng(17)
+    for a in (0.0, 0.5, 0.9, 0.998):
+        r_isco = L.orc_isco(a)
+        for r in np.linspace(r_isco * 1.001, 30.0, 25):
+            ell = _ell_kepler(r, a)
+            for xi in rng.uniform(-9, 9, 6):
+                want = L.orc_disk_g(r, a, xi)
+                got = L.orc_fluid_g(r, 0.0, a, xi, ell)
+                assert abs(got - want) <= 1e-12 * want, (a, r, xi)
+    for _ in range(400):
+        a = rng.uniform(0, 0.998)
+        r = rng.uniform(2.2, 40.0)
+        u = rng.uniform(-0.95, 0.95)
+        xi = rng.uniform(-8, 8)
+        s2 = 1 - u * u
+        sig = r * r + a * a * u * u
+        dlt = r * r - 2 * r + a * a
+        A = (r * r + a * a) ** 2 - a * a * dlt * s2
+        if dlt <= 0:
+            continue
+        om = 2 * a * r / A
+        alpha = math.sqrt(dlt * sig / A)
+        if 1 - om * xi <= 0:
+            continue
+        got = L.orc_fluid_g(r, u, a, xi, 0.0)
+        assert abs(got - alpha / (1 - om * xi)) <= 1e-12 * got
+    checked = 0
+    for _ in range(2000):
+        a = rng.uniform(0, 0.998)
+        r = rng.uniform(2.5, 40.0)
+        u = rng.uniform(-0.95, 0.95)
+        xi = rng.uniform(-8, 8)
+        ell = rng.uniform(-2, 6)
+        want = _g_from_reference_gcon(ref, r, u, a, xi, ell)
+        got = L.orc_fluid_g(r, u, a, xi, ell)
+        if want is None or want <= 0:
+            assert got == 0.0
+            continue
+        assert abs(got - want) <= 1e-10 * want, (a, r, u, xi, ell)
+        checked += 1
+    assert checked > 500
+
+
+def test_volume_point_frame_transforms_match_reference(L, ref):
+    """volume_point's observer-frame coefficients per unit affine parameter:
+    e = g^2 j_nu(nu/g) L and a = alpha_nu(nu/g) L / g, i.e. the reference's
+    grTransformEmission / grTransformAbsorption (rte_integrator.h:465-515) of
+    its synchrotronThermalEmissivity / Absorption at the fluid-frame
+    frequency, with g from the reference's inverse metric.  Uniform cells
+    (the trilinear fetch is exact on them) with the K_2 ratio 1, the
+    reference's no-Boost K_2 = 2 Theta^2; outside the support no sample is
+    taken."""
+    rng = np.random.default_rng(23)
+    n = 4
+    L_cm = 1.4766250e5
+    for _ in range(200):
+        ne, th, B = (np.float32(10 ** rng.uniform(3, 7)), np.float32(10 ** rng.uniform(-0.5, 2)),
+                     np.float32(10 ** rng.uniform(0, 2)))
+        cells = np.empty((n, n, n, 4), dtype=np.float32)
+        cells[...] = (ne, th, B, 1.0)
+        flat = np.ascontiguousarray(cells.reshape(-1))
+        a = rng.uniform(0, 0.998)
+        ell = rng.uniform(0, 4.5)
+        vol = abi.BH_Volume(None, n, n, n, 0, 2.0, 50.0, 2.5, 40.0, 0.9, ell)
+        r, u, phi = rng.uniform(3, 39), rng.uniform(-0.85, 0.85), rng.uniform(0, 6.28)
+        xi, eta = rng.uniform(-6, 6), rng.uniform(0, 30)
+        nu = 2.3e11
+        ae = np.zeros(2)
+        assert L.orc_volume_point(C.byref(vol), flat.ctypes.data_as(C.POINTER(C.c_float)),
+                                  a, xi, eta, r, u, phi, nu, L_cm, P(ae)) == 1
+        g = _g_from_reference_gcon(ref, r, u, a, xi, ell)
+        if g is None or g <= 0:
+            assert ae[0] == 0.0 and ae[1] == 0.0
+            continue
+        nf = nu / g
+        e_ref = ref.ref_gr_transform_emission(ref.ref_synch_j(nf, float(B), float(ne), float(th)), g) * L_cm
+        a_ref = ref.ref_gr_transform_absorption(ref.ref_synch_alpha(nf, float(B), float(ne), float(th)), g) * L_cm
+        assert abs(ae[1] - e_ref) <= 1e-10 * e_ref
+        assert abs(ae[0] - a_ref) <= 1e-10 * a_ref
+    vol = abi.BH_Volume(None, n, n, n, 0, 2.0, 50.0, 2.5, 40.0, 0.9, 3.0)
+    for r, u in ((2.2, 0.0), (45.0, 0.0), (10.0, 0.95)):
+        ae = np.ones(2)
+        assert L.orc_volume_point(C.byref(vol), flat.ctypes.data_as(C.POINTER(C.c_float)),
+                                  0.5, 1.0, 3.0, r, u, 0.0, 2.3e11, L_cm, P(ae)) == 0
+        assert ae[0] == 0.0 and ae[1] == 0.0
