@@ -64,10 +64,10 @@ struct SurView {
   const long long* w_off;  // per layer offsets of W_l and b_l
   const long long* b_off;
   int width, L;            // hidden width, number of weight layers (depth + 1)
-  void** render_buf;       // device scratch owned by the surrogate
-  size_t* render_cap;
 };
 SurView surrogate_view(BH_Surrogate* s);
+// Device scratch owned by the surrogate, one buffer per (device, stream).
+void* surrogate_render_scratch(BH_Surrogate* s, void* stream, size_t bytes);
 
 namespace sinf {
 
@@ -667,15 +667,9 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
   const int n_hidden = v.L - 2;
   const size_t wsw_bytes = static_cast<size_t>(n_hidden) * kW * kW * sizeof(uint16_t);
   const size_t bytes = w0_bytes + bs_bytes + wsw_bytes + hc.size() * sizeof(CamDev);
-  if (*v.render_cap < bytes) {
-    if (*v.render_buf) (void)hipFree(*v.render_buf);
-    *v.render_buf = nullptr;
-    *v.render_cap = 0;
-    const size_t cap = std::max<size_t>(bytes, w0_bytes + bs_bytes + wsw_bytes + 64 * sizeof(CamDev));
-    if (hipMalloc(v.render_buf, cap) != hipSuccess) return fail("bh_surrogate_render: hipMalloc failed");
-    *v.render_cap = cap;
-  }
-  char* scratch = static_cast<char*>(*v.render_buf);
+  const size_t cap = std::max<size_t>(bytes, w0_bytes + bs_bytes + wsw_bytes + 64 * sizeof(CamDev));
+  char* scratch = static_cast<char*>(surrogate_render_scratch(s, stream, cap));
+  if (!scratch) return fail("bh_surrogate_render: hipMalloc failed");
   uint16_t* d_w0 = reinterpret_cast<uint16_t*>(scratch);
   float* d_bs = reinterpret_cast<float*>(scratch + w0_bytes);
   uint16_t* d_wsw = reinterpret_cast<uint16_t*>(scratch + w0_bytes + bs_bytes);

@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -97,8 +99,14 @@ struct BH_Surrogate {
   uint16_t* d_t2 = nullptr;                  // transpose scratch [width][B]
   float* d_loss = nullptr;                   // 2 floats
   long long t = 0;                           // Adam step
-  void* d_render = nullptr;                  // rendering scratch (sur_infer.hip)
-  size_t render_cap = 0;
+  // Rendering scratch (sur_infer.hip), one buffer per (device, stream):
+  // renders of one surrogate on different streams never share it.
+  struct RenderBuf {
+    void* d = nullptr;
+    size_t cap = 0;
+  };
+  std::mutex render_mu;
+  std::map<std::pair<int, void*>, RenderBuf> render_bufs;
 };
 
 namespace bhk {
@@ -110,13 +118,28 @@ struct SurView {
   const long long* w_off;
   const long long* b_off;
   int width, L;
-  void** render_buf;
-  size_t* render_cap;
 };
 
 SurView surrogate_view(BH_Surrogate* s) {
-  return SurView{s->d_p, s->d_pbf, s->w_off.data(), s->b_off.data(), s->width, s->L, &s->d_render,
-                 &s->render_cap};
+  return SurView{s->d_p, s->d_pbf, s->w_off.data(), s->b_off.data(), s->width, s->L};
+}
+
+// The surrogate's rendering scratch for (current device, stream), at least
+// `bytes` long; nullptr if it cannot be allocated.  Growing a buffer frees
+// the old one (hipFree waits for the work that still reads it).
+void* surrogate_render_scratch(BH_Surrogate* s, void* stream, size_t bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(s->render_mu);
+  BH_Surrogate::RenderBuf& b = s->render_bufs[{dev, stream}];
+  if (b.cap < bytes) {
+    if (b.d) (void)hipFree(b.d);
+    b.d = nullptr;
+    b.cap = 0;
+    if (hipMalloc(&b.d, bytes) != hipSuccess) return nullptr;
+    b.cap = bytes;
+  }
+  return b.d;
 }
 
 }  // namespace bhk
@@ -303,7 +326,7 @@ void bh_surrogate_destroy(BH_Surrogate* s) {
   for (auto p : s->d_x) f(p);
   for (auto p : s->d_z) f(p);
   f(s->d_x0); f(s->d_y); f(s->d_dx); f(s->d_dz); f(s->d_dout); f(s->d_t1); f(s->d_t2); f(s->d_loss);
-  f(s->d_render);
+  for (auto& kv : s->render_bufs) f(kv.second.d);
   delete s;
 }
 

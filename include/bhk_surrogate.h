@@ -140,7 +140,11 @@ typedef struct BH_SurrogateRenderParams {
  * other rays to audit_ids; both lists are in the form bh_trace_pixel_list
  * takes.  Cameras must have r_obs = 100 M (the surrogate's training
  * distance).  All pointers except cams, opts, rp are device memory;
- * ring_tables from bh_surrogate_ring_tables, uploaded. */
+ * ring_tables from bh_surrogate_ring_tables, uploaded.
+ * Threading: one surrogate may render on several streams (and host threads)
+ * at once; its scratch (device cameras, layer-0 and bias tables, swizzled
+ * weights) is kept per (device, stream) under a lock.  Training the same
+ * surrogate while it renders is not supported. */
 int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
                         const double* disk_luts, int disk_lut_n, const float* ring_tables,
                         const BH_SurrogateRenderParams* rp, BH_PixelRecord* out,

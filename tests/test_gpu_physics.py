@@ -64,7 +64,9 @@ def test_backend_abi_renders_into_its_framebuffer():
         assert ptr and (w.value, h.value) == (32, 24)
         # the backend's framebuffer equals a direct launch of the same params and variant
         got = torch.empty((24, 32, 4), dtype=torch.float32, device="cuda")
-        hip = C.CDLL("libamdhip64.so")
+        # the HIP runtime torch and libbhk.so already share (RTLD_NOLOAD: never map a second copy)
+        import os
+        hip = C.CDLL("libamdhip64.so.7", mode=os.RTLD_NOLOAD | os.RTLD_LAZY)
         hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         assert hip.hipMemcpy(C.c_void_p(got.data_ptr()), C.c_void_p(ptr), got.numel() * 4, 3) == 0  # D2D
         ref = tracer.render_launch(p, variant=2)
