@@ -55,6 +55,8 @@ def lib():
         C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions),
         C.c_void_p, C.c_int, C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p,
     ]
+    L.bh_trace_image_plane_noalloc.argtypes = L.bh_trace_image_plane.argtypes
+    L.bh_trace_image_plane_noalloc.restype = C.c_int
     L.bh_trace_pixel_list.argtypes = [
         C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions), C.c_void_p, C.c_int,
         C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]

@@ -665,6 +665,68 @@ int launch_on_stream(const LaunchImage& im, void* stream, const char* who) {
   return 0;
 }
 
+// Allocation-free launches (bh_trace_image_plane_noalloc): the queue head
+// and the launch block live in this file-scope device array, one copy per
+// GPU, held until the process ends.  Layout: [0, 256) the ray-queue head,
+// [256, kNoAllocBytes) the launch block.  Every call on a GPU shares its
+// copy, so calls that overlap on one GPU (different streams or threads)
+// race; calls on one stream are ordered by the stream.
+constexpr size_t kNoAllocBytes = 64 * 1024;
+constexpr size_t kNoAllocHead = 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_ptr[64] = {};  // device address of scratch, per device
+
+// Device address of scratch on the current device, looked up once per device
+// with that device current.  The array's name is a host-side stand-in and is
+// never passed to the device.
+unsigned char* noalloc_scratch(const char* who) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    fail(std::string(who) + ": no HIP device");
+    return nullptr;
+  }
+  if (dev < 0 || dev >= 64) {
+    fail(std::string(who) + ": device index beyond 63");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fail(std::string(who) + ": hipGetSymbolAddress(scratch) failed: " + hipGetErrorString(e));
+      return nullptr;
+    }
+    g_noalloc_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_noalloc_ptr[dev];
+}
+
+// Uploads a launch image into the device's scratch array and launches it:
+// no device allocation and no event.
+int launch_noalloc(const LaunchImage& im, void* stream, const char* who) {
+  const size_t bytes = im.bytes();
+  if (bytes > kNoAllocBytes - kNoAllocHead)
+    return fail(std::string(who) + ": launch block of " + std::to_string(bytes) + " bytes exceeds the " +
+                std::to_string(kNoAllocBytes - kNoAllocHead) + "-byte scratch (at most " +
+                std::to_string((kNoAllocBytes - kNoAllocHead - sizeof(DevParams)) / sizeof(CamDev)) +
+                " cameras without a volume set)");
+  unsigned char* base = noalloc_scratch(who);
+  if (!base) return -1;
+  unsigned long long* d_head = reinterpret_cast<unsigned long long*>(base);
+  unsigned char* d_block = base + kNoAllocHead;
+  thread_local unsigned char host_img[kNoAllocBytes];
+  im.serialize(host_img, d_block, d_head);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // Pageable source: the runtime stages it before returning, so host_img may
+  // be reused by the next call on this thread.
+  if (hipMemcpyAsync(d_block, host_img, bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail(std::string(who) + ": parameter upload failed");
+  return launch_image(im, reinterpret_cast<const DevParams*>(d_block), d_head, s, who);
+}
+
 int trace_launch(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts, const double* disk_luts,
                  int disk_lut_n, const BH_Volume* vol, const BH_VolumeSet* vs, const BH_RaySpec* rays,
                  long long n_rays, const int64_t* ids, const uint64_t* n_ids_dev, long long max_ids,
@@ -799,6 +861,17 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cams, const BH_TraceOption
                          BH_PixelRecord* out, void* stream) {
   return trace_launch(cams, n_cams, opts, disk_luts, disk_lut_n, vol, nullptr, nullptr, 0, nullptr, nullptr, 0, out,
                       stream, false, "bh_trace_image_plane");
+}
+
+int bh_trace_image_plane_noalloc(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
+                                 const double* disk_luts, int disk_lut_n, const BH_Volume* vol,
+                                 BH_PixelRecord* out, void* stream) {
+  const char* who = "bh_trace_image_plane_noalloc";
+  LaunchImage im;
+  if (build_launch(cams, n_cams, opts, disk_luts, disk_lut_n, vol, nullptr, nullptr, 0, nullptr, nullptr, 0, out,
+                   false, who, &im) != 0)
+    return -1;
+  return launch_noalloc(im, stream, who);
 }
 
 int bh_trace_image_plane_pol(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts, const double* disk_luts,

@@ -240,6 +240,22 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cameras,
                          const BH_Volume* vol,
                          BH_PixelRecord* out, void* stream);
 
+/* bh_trace_image_plane without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and records as bh_trace_image_plane;
+ * the launch block and the ray-queue head live in a 64 KiB device array that
+ * the library holds once per GPU until the process ends (at most 540
+ * cameras per call; larger launches return -1 with the limit in
+ * bh_last_error()).  All calls on one GPU share that array: calls on one
+ * stream are ordered, calls that overlap on one GPU (other streams or host
+ * threads) race.  bh_last_kernel_ms() does not time these launches. */
+int bh_trace_image_plane_noalloc(const BH_Camera* cams, int n_cameras,
+                                 const BH_TraceOptions* opts,
+                                 const double* disk_luts, int disk_lut_n,
+                                 const BH_Volume* vol,
+                                 BH_PixelRecord* out, void* stream);
+
 /* Trace only the pixels whose global record ids are listed (device int64
  * ids[*n_ids_dev], count in device memory so a producer kernel can append
  * ids without a host round trip; at most max_ids).  Record of id i goes to
