@@ -32,7 +32,13 @@ Other workloads (one JSON line each, same contract):
                  persistent launch replayed from a hipGraph, autotuned shape;
   --workload c3  the 4096^2 precision form (FP32 + FP64 islands) against FP64;
   --workload c2  the volumetric FP64 DP45 tracer on the c2 frame (1024^2,
-                 a = 0.94, 17 deg, torus 256 x 128 x 64, 230 GHz).
+                 a = 0.94, 17 deg, torus 256 x 128 x 64, 230 GHz);
+  --workload c1  the 512^2 FP64 RK4 thin-disk frame against the CPU oracle
+                 over the full frame.
+
+The CPU baseline (``cpu_baseline``) is the FP64 oracle on min(16, affinity)
+host threads: one warm-up pass and three timed passes over a bounded sample
+of the workload, median reported, with the CPU model and affinity count.
 
 Every roofline is priced against MI355X's published peak (``peak``, ``frac``);
 after the workload the library's micro-kernels measure the device's
@@ -40,7 +46,7 @@ attainable FP64 / FP32 FMA, BF16 MFMA and copy rates (``measured_peaks``),
 and each roofline also carries ``peak_measured`` and ``frac_measured``
 (``--no-peaks`` skips them).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c4|c3|c2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c4|c3|c2|c1]
 """
 from __future__ import annotations
 
@@ -67,25 +73,78 @@ BF16_DENSE_PEAK_TFLOPS = 2500.0
 WEIGHTS = os.path.join(ROOT, "blackhole_amd", "weights", "surrogate_c5.npz")
 
 
-def cpu_baseline(w, gpu_rec_np_fn, ids, label: str):
-    """Time the FP64 CPU oracle on the listed pixels and check
-    the GPU records at those pixels against it."""
+def cpu_threads() -> int:
+    """Host threads the CPU baseline uses: the GPU's share of the host (16),
+    never more than this process may run on."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def host_cpu() -> dict:
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"affinity_cpus": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count(), "cpu_model": model}
+
+
+def time_cpu(fn, reps: int = 3):
+    """One warm-up call, then `reps` timed calls; returns (result of the
+    warm-up, warm-up s, list of timed s, median s)."""
+    import statistics
+    t0 = time.perf_counter()
+    out = fn()
+    warm = time.perf_counter() - t0
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return out, warm, times, statistics.median(times)
+
+
+def cpu_baseline(w, gpu_rec_np_fn, ids, label: str, reps: int = 3):
+    """Time the FP64 CPU oracle on the listed pixels (1 warm-up + `reps`
+    passes, median) on cpu_threads() threads, and check the GPU records at
+    those pixels against it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     from parity_util import compare, run_oracle
-    threads = min(16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    ref = run_oracle(oracle_lib, w, ids=ids)
-    dt = time.perf_counter() - t0
+    threads = cpu_threads()
+    ref, warm, times, med = time_cpu(lambda: run_oracle(oracle_lib, w, ids=ids, nthreads=threads), reps)
     par = compare(gpu_rec_np_fn(ids), ref)
+    return cpu_record(ids.size, threads, label, warm, times, med), par
+
+
+def cpu_record(n: int, threads: int, label: str, warm: float, times, med: float) -> dict:
     return {
-        "value": ids.size / dt,
+        "value": n / med,
         "unit": "rays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{label} ({ids.size} rays, {dt:.1f} s wall on {threads} threads, FP64 oracle, "
-                  f"-O3 -march=native)",
-    }, par
+        "sample": f"{label} ({n} rays; 1 warm-up + {len(times)} timed passes, median {med:.2f} s on {threads} "
+                  f"threads; FP64 oracle, -O3 -march=native)",
+        "warmup_s": warm, "timed_s": times, **host_cpu(),
+    }
+
+
+def drift_summary(rec) -> dict:
+    """Conserved-quantity drift of a traced image (device records): the null
+    condition and the polar (Carter) first integral at each ray's exit state,
+    |v_r^2 - R(r)| / (r^2 + a^2)^2 and |v_u^2 - U(u)| / (|eta| + a^2 + xi^2).
+    E and L are exact by construction (constants of the integrated form)."""
+    import torch
+    d = rec.view(torch.float64)
+    m = rec[:, 112] != 0
+    dn, dc = d[:, 11][m], d[:, 12][m]
+    return {"rays": int(m.sum()), "max_null": float(dn.max()), "mean_null": float(dn.mean()),
+            "max_carter": float(dc.max()), "mean_carter": float(dc.mean()),
+            "frac_null_gt_1e-9": float((dn > 1e-9).double().mean()),
+            "frac_carter_gt_1e-9": float((dc > 1e-9).double().mean())}
 
 
 def time_steps(fn, steps: int, warmup: int):
@@ -108,10 +167,14 @@ def time_steps(fn, steps: int, warmup: int):
 
 def frame_accuracy(sur_rec, plain_rec, cams):
     """Per-frame and pooled accuracy of the surrogate records against the
-    plain ones, on the device (records are uint8 [n, 128] tensors)."""
+    plain ones, on the device (records are uint8 [n, 128] tensors).  The
+    photon-ring diameter of both images comes from the same shadow-edge
+    estimator (analysis.ring_diameter_errors), which also checks the plain
+    image's ring against the closed-form Bardeen curve; the older
+    area-equivalent diameter of the captured region is reported beside it."""
     import math
     import torch
-    from blackhole_amd import abi
+    from blackhole_amd import abi, analysis
     I_s = sur_rec.view(torch.float64)[:, 0]
     I_p = plain_rec.view(torch.float64)[:, 0]
     c_s = sur_rec[:, 112]
@@ -126,12 +189,17 @@ def frame_accuracy(sur_rec, plain_rec, cams):
         pix = (cam.alpha_max - cam.alpha_min) / cam.width * (cam.beta_max - cam.beta_min) / cam.height
         d_s = 2.0 * math.sqrt(float((c_s[a:b] == abi.HIT_HORIZON).sum()) * pix / math.pi)
         d_p = 2.0 * math.sqrt(float((c_p[a:b] == abi.HIT_HORIZON).sum()) * pix / math.pi)
+        ring = analysis.ring_diameter_errors(c_s[a:b].cpu().numpy(), c_p[a:b].cpu().numpy(), cam)
         f = {
             "spin": cam.spin, "theta_deg": round(math.degrees(cam.theta_obs), 3),
             "mean_rel": float(rel.mean()), "max_rel": float(rel.max()),
             "frac_rel_gt_1e-2": float((rel > 1e-2).double().mean()),
             "flux_weighted": float((is_ - ip).abs().sum() / ip.sum()),
-            "ring_diameter_rel_err": abs(d_s - d_p) / d_p,
+            "ring_diameter_rel_err": ring["rel_err"],
+            "ring_measured_on": ring["measured_on"], "ring_bins": ring["ring_bins"],
+            "ring_diameter_M": ring["diameter_ref"],
+            "plain_ring_vs_bardeen_rel": ring.get("ref_vs_curve_rel"),
+            "shadow_area_diameter_rel_err": abs(d_s - d_p) / d_p,
             "class_disagreements": int((c_s[a:b] != c_p[a:b]).sum()),
         }
         frames.append(f)
@@ -144,6 +212,10 @@ def frame_accuracy(sur_rec, plain_rec, cams):
         "flux_weighted": tot_abs / tot_ref,
         "max_frame_mean_rel": max(f["mean_rel"] for f in frames),
         "max_ring_diameter_rel_err": max(f["ring_diameter_rel_err"] for f in frames),
+        "max_plain_ring_vs_bardeen_rel": max((f["plain_ring_vs_bardeen_rel"] for f in frames
+                                              if f["plain_ring_vs_bardeen_rel"] is not None), default=None),
+        "frames_ring_visible": sum(1 for f in frames if f["ring_measured_on"] == "ring"),
+        "max_shadow_area_diameter_rel_err": max(f["shadow_area_diameter_rel_err"] for f in frames),
         "class_disagreements": sum(f["class_disagreements"] for f in frames),
         "frames": frames,
     }
@@ -315,6 +387,23 @@ def bench_c5(args):
             "fallback_rays": fbm, "newton_iters_per_ray": it_mino, "gate_pass": passes(accm),
             "accuracy_vs_plain_fp64": summ(accm),
         },
+        "learned_part_net_effect": {
+            "note": "the surrogate pipeline (network + exact refinement from its radii + fallback + audit) "
+                    "against the network-free pipeline (exact solve from a cubic Hermite start + fallback) "
+                    "at the same accuracy gate; ratio < 1 is a net loss of the learned stage",
+            "surrogate_pipeline_ms": sur_ms, "network_free_pipeline_ms": mino_ms,
+            "ratio_network_free_ms_over_surrogate_ms": mino_ms / sur_ms,
+            "newton_iters_from_network": it_sur, "newton_iters_from_hermite": it_mino,
+            "refine_from_network_ms": refine_render_ev_ms - render_ev_ms,
+            "solve_from_hermite_ms": mino_solve_ev_ms,
+        },
+        "ceiling_at_full_bf16_peak": {
+            "network_ms_at_2500_tflops": f_sur / (BF16_DENSE_PEAK_TFLOPS * 1e12) * 1000.0,
+            "speedup_vs_plain_if_network_alone_at_peak": plain_ms / (f_sur / (BF16_DENSE_PEAK_TFLOPS * 1e12) * 1000.0),
+            "note": "the 6 x 256 SiLU network costs 0.672 MFLOP per ray; even at 100 % of the dense BF16 peak and "
+                    "with nothing else in the pipeline it cannot reach 10x the plain integrator on this workload",
+        },
+        "plain_drift": drift_summary(plain),
         "frames": acc["frames"],
     }
     if not args.no_cpu_baseline:
@@ -340,7 +429,7 @@ def bench_c4(args):
     frame replayed from its hipGraph in the timed region."""
     import numpy as np
     import torch
-    from blackhole_amd import tracer, workloads
+    from blackhole_amd import flops, tracer, workloads
 
     w = workloads.c4()
     sc = tracer.upload_scene(w["cams"], w["opts"], luts=w["luts"])
@@ -374,7 +463,9 @@ def bench_c4(args):
         "graph_vs_eager": {"graph_ms": graph_ms, "eager_ms": eager_ms, "graph_event_ms": graph_ev,
                            "eager_event_ms": eager_ev},
         "frame_stats": {"attempts_mean": attempts / n, "samples_mean": samples / n,
-                        "volume_GBps_algorithmic": samples * 128 / (graph_ev / 1000.0) / 1e9},
+                        "volume_GBps_algorithmic": samples * flops.VOLUME_SAMPLE_BYTES / (graph_ev / 1000.0) / 1e9,
+                        "volume_GBps_stored_layout": samples * flops.VOLUME_SAMPLE_BYTES_STORED
+                        / (graph_ev / 1000.0) / 1e9},
     }
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -386,20 +477,22 @@ def bench_c4(args):
         offs = np.array([c.record_offset for c in w["cams"]])
         cam_of = np.searchsorted(offs, ids, side="right") - 1
         gpu = tracer.records_to_numpy(out[torch.from_numpy(ids).cuda()])
-        threads = min(16, os.cpu_count() or 1)
-        t0 = time.perf_counter()
-        pars = []
+        threads = cpu_threads()
+        buckets = []
         for b in range(len(w["vols"])):
             sel = np.nonzero(np.isin(cam_of, [i for i, v in enumerate(w["cam_vol"]) if v == b]))[0]
-            if sel.size == 0:
-                continue
-            ref = oracle_lib.trace(w["cams"], w["opts"], luts=w["luts"], lut_n=w["luts"].shape[1], vol=w["vols"][b],
-                                   cells=w["cells_list"][b].reshape(-1), ids=ids[sel], nthreads=threads)
-            pars.append(compare(gpu[sel], ref))
-        dt = time.perf_counter() - t0
-        res["cpu_baseline"] = {"value": m / dt, "unit": "rays/s", "cores": threads, "kind": "port",
-                               "sample": f"{m} seeded random pixels of the 256 c4 cameras ({dt:.1f} s wall on "
-                                         f"{threads} threads, FP64 oracle per spin bucket)"}
+            if sel.size:
+                buckets.append((b, sel))
+
+        def trace_sample():
+            return [(sel, oracle_lib.trace(w["cams"], w["opts"], luts=w["luts"], lut_n=w["luts"].shape[1],
+                                           vol=w["vols"][b], cells=w["cells_list"][b].reshape(-1), ids=ids[sel],
+                                           nthreads=threads)) for b, sel in buckets]
+
+        refs, warm, times, med = time_cpu(trace_sample)
+        pars = [compare(gpu[sel], ref) for sel, ref in refs]
+        res["cpu_baseline"] = cpu_record(m, threads, f"{m} seeded random pixels of the 256 c4 cameras, FP64 oracle "
+                                                     f"per spin bucket", warm, times, med)
         res["plain_gpu_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
         res["parity_vs_oracle_on_sample"] = {
             "n": sum(p["n"] for p in pars), "class_mismatch": sum(p["class_mismatch"] for p in pars),
@@ -457,6 +550,50 @@ def bench_c3(args):
     }
 
 
+def bench_c1(args):
+    """Plain method, small: the 512^2 fixed-step FP64 RK4 frame (a = 0.9,
+    60 deg, r_obs 100 M, thin NT disk r_isco -> 20 M) on the GPU, against
+    the FP64 CPU oracle over the full frame on the host's cores."""
+    import numpy as np
+    import torch
+    from blackhole_amd import abi, flops, tracer, workloads
+
+    size = args.size if args.size != 2048 else 512
+    w = workloads.c1(size, size)
+    sc = tracer.upload_scene(w["cams"], w["opts"], luts=w["luts"])
+    out = tracer.alloc_records(sc.n_records)
+    stream = torch.cuda.current_stream()
+    ms, ev = time_steps(lambda: tracer.launch(sc, out, stream), args.steps, args.warmup)
+    tracer.launch(sc, out, stream)
+    torch.cuda.synchronize()
+    k_ms = tracer.last_kernel_ms()
+    rec = tracer.records_to_numpy(out)
+    f = flops.frame_flops(rec, abi.INTEGRATOR_RK4)
+    achieved = f / (k_ms / 1000.0) / 1e12
+    n = sc.n_records
+    res = {
+        "metric": METRIC, "value": n / (ms / 1000.0), "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (analytic camera and disk)",
+        "config": {"workload": f"c1: plain GPU RK4 FP64, a=0.9, 60 deg, r_obs 100 M, {size}x{size} over "
+                               f"alpha,beta in [-15,15] M, h=0.01, <=2000 steps, thin NT disk r_isco-20 M",
+                   "rays": n},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_trace_fp64<RK4, no volume>", "kernel_ms": k_ms, "flop_per_launch": f},
+        "frame_stats": {"steps_mean": float(rec["steps"].mean()),
+                        "classes": np.bincount(rec["hit_class"], minlength=6).tolist()},
+        "drift": drift_summary(out),
+    }
+    if not args.no_cpu_baseline:
+        ids = np.arange(n, dtype=np.int64)
+        cb, par = cpu_baseline(w, lambda sel: rec[sel], ids, f"the full {size}x{size} c1 frame")
+        res["cpu_baseline"] = cb
+        res["plain_gpu_vs_cpu"] = res["value"] / cb["value"]
+        res["parity_vs_oracle_full_frame"] = par
+    return res
+
+
 def bench_c2(args):
     import numpy as np
     import torch
@@ -511,8 +648,11 @@ def bench_c2(args):
             "samples_mean": float(rec["n_samples"].mean()),
             "volume_GBps_algorithmic": float(rec["n_samples"].astype(np.float64).sum()
                                              * flops.VOLUME_SAMPLE_BYTES / (k_ms / 1000.0) / 1e9),
+            "volume_GBps_stored_layout": float(rec["n_samples"].astype(np.float64).sum()
+                                               * flops.VOLUME_SAMPLE_BYTES_STORED / (k_ms / 1000.0) / 1e9),
             "classes": np.bincount(rec["hit_class"], minlength=6).tolist(),
         },
+        "drift": drift_summary(out),
     }
     if not args.no_cpu_baseline:
         n = sc.n_records
@@ -562,7 +702,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c5", "c4", "c3", "c2"], default="c5")
+    ap.add_argument("--workload", choices=["c5", "c4", "c3", "c2", "c1"], default="c5")
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=8_000_000, help="c5: oracle pixels for the CPU baseline")
@@ -580,7 +720,7 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
-    res = {"c5": bench_c5, "c4": bench_c4, "c3": bench_c3, "c2": bench_c2}[args.workload](args)
+    res = {"c5": bench_c5, "c4": bench_c4, "c3": bench_c3, "c2": bench_c2, "c1": bench_c1}[args.workload](args)
     if not args.no_peaks:
         attach_measured_peaks(res, measured_peaks())
     print(json.dumps(res), flush=True)

@@ -6,10 +6,16 @@
 //     constants and right-hand side in FP32, the state advanced by
 //     Kahan-compensated sums of the step increments;
 //   * FP64 islands: the ray setup (Carter constant and impact parameters),
-//     every step that starts inside r < island_r_factor * r_+, and every step
+//     every step that starts inside r < island_r_factor * r_+, every step
 //     that starts within island_turn_dist (M) of a radial turning point
 //     (v_r^2 / |R'(r)|) or a polar one (r |delta u| / sin(theta) with
-//     delta u = v_u^2 / |U'(u)|);
+//     delta u = v_u^2 / |U'(u)|), and every step that starts within
+//     kPoleIsland of the polar axis (sin^2 theta < 1e-3): there
+//     dphi/dlambda = xi / sin^2 theta is singular, FP32 cannot resolve
+//     sin^2 theta = 1 - u^2 of a ray that turns close to the axis, and its
+//     step-size control would stall.  An island step's FP64 result is kept
+//     in the compensated pair (y, cy), so consecutive island steps carry
+//     the FP64 state without rounding it to FP32;
 //   * FP32 radiative transfer with Kahan-compensated optical depth and
 //     intensity, in host-prescaled units (every intermediate in FP32 range).
 // Replaces the reference's FP32 render kernel (geodesicTraceFp32,
@@ -77,10 +83,14 @@ __device__ __forceinline__ bool lane_setup(LaneF& L, const DevParams& P, long lo
   return true;
 }
 
+// sin^2 theta below which steps run in FP64 (theta within ~1.8 deg of the axis).
+constexpr float kPoleIsland = 1e-3f;
+
 // FP64 island test at the start of a step.
 __device__ __forceinline__ bool needs_island(const LaneF& L, const DevParams& P) {
   const float r = L.y[0], u = L.y[1];
   if (r < L.island_r) return true;
+  if (fmaT(-u, u, 1.0f) < kPoleIsland) return true;
   const float D = static_cast<float>(P.island_turn_dist);
   const float Pp = r * r + L.cf.c1;
   const float Rp = fmaT(4.0f * r, Pp, -((2.0f * r - 2.0f) * L.cf.K));  // R'(r)
@@ -409,8 +419,9 @@ __device__ __forceinline__ int step_f(LaneF& L, const DevParams& P) {
     L.fp64_steps++;
 #pragma unroll
     for (int i = 0; i < kN; ++i) {
+      // y - cy carries the FP64 result to ~2^-48: cy is the rounding residual
       L.y[i] = static_cast<float>(y1d[i]);
-      L.cy[i] = 0.0f;
+      L.cy[i] = static_cast<float>(static_cast<double>(L.y[i]) - y1d[i]);
       L.k1[i] = static_cast<float>(f1d[i]);
     }
     if (VOL) volume_segment_f(L, P, static_cast<float>(hd));

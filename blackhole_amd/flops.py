@@ -15,7 +15,11 @@ VOLUME_SAMPLE = 186     # trilinear fetch (4 fields) + fluid g + Planck + j + a,
 VOLUME_STEP_NOSAMPLE = 7  # affine rate + segment bookkeeping on accepted steps without a sample
 SETUP = 65              # image plane -> constants + first RHS
 
-VOLUME_SAMPLE_BYTES = 128  # algorithmic bytes per trilinear sample: 8 corners x 4 FP32 fields
+# Algorithmic bytes per trilinear sample: 8 corners x 3 FP32 fields (n_e, Theta_e, |B|).
+VOLUME_SAMPLE_BYTES = 96
+# Bytes the kernel reads per sample: each corner is one 16-B float4 whose fourth
+# channel holds the precomputed K_2(1/Theta_e) / (2 Theta_e^2) ratio.
+VOLUME_SAMPLE_BYTES_STORED = 128
 
 
 def frame_flops(rec, integrator: int) -> float:

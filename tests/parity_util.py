@@ -20,11 +20,13 @@ def run_gpu(w: dict) -> np.ndarray:
     return tracer.records_to_numpy(out)
 
 
-def run_oracle(oracle, w: dict, ids=None) -> np.ndarray:
+def run_oracle(oracle, w: dict, ids=None, nthreads: int = 0) -> np.ndarray:
+    """Oracle records of workload w (all rays, or the listed ids); nthreads 0
+    lets the oracle use every hardware thread."""
     cells = w.get("cells")
     return oracle.trace(
         w["cams"], w["opts"], luts=w.get("luts"), lut_n=(w["luts"].shape[-1] if w.get("luts") is not None else 0),
-        vol=w.get("vol"), cells=cells.reshape(-1) if cells is not None else None, ids=ids)
+        vol=w.get("vol"), cells=cells.reshape(-1) if cells is not None else None, ids=ids, nthreads=nthreads)
 
 
 def rel(a, b):
@@ -48,6 +50,9 @@ def compare(gpu: np.ndarray, ref: np.ndarray, tol: float = 1e-9) -> dict:
     for f in ("r_exit", "u_exit", "phi_exit", "intensity", "tau"):
         r = rel(gpu[f][m], ref[f][m])
         out[f"max_rel_{f}"] = float(r.max()) if r.size else 0.0
+    for f in ("drift_null", "drift_carter"):
+        d = np.abs(gpu[f][m] - ref[f][m])
+        out[f"max_abs_{f}_diff"] = float(d.max()) if d.size else 0.0
     out["steps_mismatch"] = int((gpu["steps"][m] != ref["steps"][m]).sum())
     out["attempts_mismatch"] = int((gpu["attempts"][m] != ref["attempts"][m]).sum())
     return out

@@ -48,3 +48,47 @@ def test_intensity_errors():
     ref = np.array([1.0, 2.0, 0.0, 4.0])
     e = analysis.intensity_errors(ref * (1 + 1e-4), ref)
     assert abs(e["mean_rel"] - 1e-4) < 1e-12 and e["n_pixels"] == 3
+
+
+def _inside_curve_image(cam, scale=1.0, covered=None):
+    """Hit-class image that is HORIZON strictly inside the closed-form
+    critical curve (scaled about its centroid) and ESCAPED outside; polar
+    angles in `covered` (a (lo, hi) range in rad) are DISK (a disk in front)."""
+    ca, cb = analysis.critical_curve(cam.spin, cam.theta_obs, 20000)
+    c = analysis.curve_centre(cam.spin, cam.theta_obs)
+    psi_c = np.mod(np.arctan2(cb - c[1], ca - c[0]), 2 * math.pi)
+    rho_c = np.hypot(ca - c[0], cb - c[1]) * scale
+    o = np.argsort(psi_c)
+    A, B = analysis.pixel_grid(cam)
+    psi = np.mod(np.arctan2(B - c[1], A - c[0]), 2 * math.pi)
+    rho = np.hypot(A - c[0], B - c[1])
+    r_at = np.interp(psi, psi_c[o], rho_c[o], period=2 * math.pi)
+    img = np.where(rho < r_at, abi.HIT_HORIZON, abi.HIT_ESCAPED).astype(np.uint8)
+    if covered is not None:
+        img[(psi > covered[0]) & (psi < covered[1]) & (rho > 0.5 * r_at)] = abi.HIT_DISK
+    return img
+
+
+def test_ring_diameter_estimator_on_synthetic_images():
+    """The shadow-edge ring estimator: on an image that is captured exactly
+    inside the Bardeen curve it reproduces the curve's diameter to within a
+    pixel; it sees a 1 % larger ring as 1 %; bins where a disk covers the
+    curve are left out of the ring; with the whole curve covered it falls
+    back to the shadow outline."""
+    for a, th in ((0.0, 17.0), (0.9, 60.0), (0.998, 85.0)):
+        cam = tracer.camera(a, th, 100.0, 1024, 1024, 10.0)
+        px = 20.0 / 1024
+        img = _inside_curve_image(cam)
+        e = analysis.ring_diameter_errors(img, img, cam)
+        assert e["measured_on"] == "ring" and e["ring_bins"] == 64 and e["rel_err"] == 0.0
+        assert abs(e["diameter_ref"] - e["curve_diameter"]) <= 2 * px, e
+        big = _inside_curve_image(cam, 1.01)
+        e = analysis.ring_diameter_errors(big, img, cam)
+        assert abs(e["rel_err"] - 0.01) <= 2 * px / e["diameter_ref"], e
+        part = _inside_curve_image(cam, covered=(0.5, 2.0))
+        e = analysis.ring_diameter_errors(part, part, cam)
+        assert 40 <= e["ring_bins"] < 64 and abs(e["diameter_ref"] - e["curve_diameter"]) <= 2 * px, e
+        hidden = _inside_curve_image(cam, covered=(-1.0, 7.0))
+        e = analysis.ring_diameter_errors(hidden, hidden, cam)
+        assert e["ring_bins"] == 0 and e["measured_on"] == "shadow"
+        assert abs(e["diameter"] - 0.5 * analysis.diameter_from_radii(analysis.curve_radii(cam))) <= 4 * px

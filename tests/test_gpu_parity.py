@@ -27,6 +27,10 @@ def _check(gpu, ref, label):
     for f in ("r_exit", "u_exit", "phi_exit", "intensity", "tau"):
         assert s[f"max_rel_{f}"] <= TOL, (f, s)
     assert s["steps_mismatch"] == 0 and s["attempts_mismatch"] == 0, s
+    # the conserved-quantity drift of each record (null condition v_r^2 = R
+    # and the polar first integral v_u^2 = U at the exit state) is computed
+    # from the same final state on both sides
+    assert s["max_abs_drift_null_diff"] <= 1e-12 and s["max_abs_drift_carter_diff"] <= 1e-12, s
     return s
 
 

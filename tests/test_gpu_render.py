@@ -35,9 +35,26 @@ def test_fp32_variant_classes_and_framebuffer_shading():
                              background_enabled=1, background_intensity=0.25)
     rec64 = tracer.records_to_numpy(tracer.trace_launch_params(p, variant=2))
     rec32 = tracer.records_to_numpy(tracer.trace_launch_params(p, variant=0))
-    agree = float((rec64["hit_class"] == rec32["hit_class"]).mean())
-    print("fp32 class agreement", agree, flush=True)
-    assert agree >= 0.99
+    # hit classes identical outside the epsilon band: rays whose constants lie
+    # within 0.1 M of the critical curve (in the distant-observer image plane
+    # of the camera's inclination) or with a crossing within 1e-4 of a disk edge
+    import math
+    from blackhole_amd import analysis
+    diff = np.nonzero(rec64["hit_class"] != rec32["hit_class"])[0]
+    cam_r = math.sqrt(sum(x * x for x in (0.0, -20.0, 12.0)))
+    th = math.acos(12.0 / cam_r)
+    outside = 0
+    for i in diff:
+        _, xi, eta = OL.pinhole_setup(p, int(i % p.width), int(i // p.width))
+        al = -xi / math.sin(th)
+        be = math.sqrt(max(eta + 0.81 * math.cos(th) ** 2 - xi * xi / math.tan(th) ** 2, 0.0))
+        near_curve = analysis.curve_distance_points(tracer.camera(0.9, math.degrees(th), cam_r, 1, 1),
+                                                    np.array([al]), np.array([be]))[0] <= 0.1
+        edge = any(analysis.near_disk_edge(r["cross_r"][i], r["n_cross"][i], tracer.isco(0.9), 60.0, 1e-4)[0]
+                   for r in (rec64, rec32))
+        outside += int(not (near_curve or edge))
+    print("fp32 variant class differences", len(diff), "outside the band", outside, flush=True)
+    assert outside == 0
     fb = tracer.render_launch(p, variant=2)
     torch.cuda.synchronize()
     fb = fb.cpu().numpy().reshape(-1, 4)
