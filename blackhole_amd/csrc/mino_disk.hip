@@ -75,7 +75,10 @@ __global__ void __launch_bounds__(256, 4) k_mino_disk(const MinoParams P) {
           r_exit = (s.cls == mino::kHorizon) ? k.r_stop : P.r_escape;
         }
         d2[0] = make_double2(I, r_exit);
-        if (!P.refine) d2[1] = make_double2(0.0, 0.0);  // the escape direction stays the network's
+        // the solve finds crossings and the class, not the path's end point:
+        // u_exit and phi_exit are marked unknown (NaN), also over a network's
+        // escape-direction outputs
+        d2[1] = make_double2(__builtin_nan(""), __builtin_nan(""));
         d2[2] = make_double2(s.r[0], s.r[1]);
         d2[3] = make_double2(s.r[2], g[0]);
         d2[4] = make_double2(g[1], g[2]);

@@ -384,7 +384,9 @@ int bh_trace_image_plane_pol(const BH_Camera* cams, int n_cams, const BH_TraceOp
  * first three crossing radii, Keplerian g, LUT intensity; steps = Newton
  * iterations, flags BH_FLAG_REFINED).  Same cameras, options and LUTs as
  * bh_trace_image_plane with disk_enabled = 1 and volume_enabled = 0; the
- * escape direction (u_exit, phi_exit) is not computed (0).  Rays the solver
+ * path's end point direction (u_exit, phi_exit) is not computed and is
+ * written as NaN (also in records this solver refines, over a surrogate's
+ * escape-direction outputs).  Rays the solver
  * does not resolve (within about 1e-3 M of the critical curve) get
  * BH_FLAG_FALLBACK and their ids in fallback_ids (count in device memory,
  * zeroed by the call) for bh_trace_pixel_list.  No counterpart in the
