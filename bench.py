@@ -472,7 +472,7 @@ def bench_c4(args):
         import oracle_lib
         from parity_util import compare
         rng = np.random.default_rng(20261204)
-        m = min(args.cpu_sample // 8, n)
+        m = min(args.cpu_sample, n)
         ids = np.sort(rng.choice(n, size=m, replace=False)).astype(np.int64)
         offs = np.array([c.record_offset for c in w["cams"]])
         cam_of = np.searchsorted(offs, ids, side="right") - 1
@@ -531,7 +531,7 @@ def bench_c3(args):
     fp64_frac = float(w32[:, 30].double().sum() / w32[:, 26].double().sum())
     f, _ = device_frame_flops(recs["fp32"])
     achieved = f / (res_legs["fp32"]["kernel_ms"] / 1000.0) / 1e12
-    return {
+    res = {
         "metric": METRIC, "value": res_legs["fp32"]["rays_per_s"], "unit": "rays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": res_legs["fp32"]["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -548,6 +548,19 @@ def bench_c3(args):
         "rays_with_fp64_islands": islands, "fp64_step_fraction": fp64_frac,
         "critical_curve_band_M": 0.1,
     }
+    if not args.no_cpu_baseline:
+        import numpy as np
+        w["opts"].precision = abi.PRECISION_FP64
+        rng = np.random.default_rng(20261205)
+        ids = np.sort(rng.choice(n, size=min(n, max(1, n // 16)), replace=False)).astype(np.int64)
+        rec64 = tracer.records_to_numpy(recs["fp64"][torch.from_numpy(ids).cuda()])
+        cb, par = cpu_baseline(w, lambda sel: rec64, ids,
+                               f"{ids.size} seeded random pixels (1/16) of the {size}x{size} c3 frame, FP64")
+        res["cpu_baseline"] = cb
+        res["fp64_gpu_vs_cpu"] = res_legs["fp64"]["rays_per_s"] / cb["value"]
+        res["fp32_gpu_vs_cpu"] = res_legs["fp32"]["rays_per_s"] / cb["value"]
+        res["fp64_parity_vs_oracle_on_sample"] = par
+    return res
 
 
 def bench_c1(args):

@@ -213,11 +213,19 @@ __device__ __forceinline__ int step_dp45(Lane& L, const DevParams& P) {
   return 0;
 }
 
-// Two workgroups per CU: the volume variants would otherwise take a few
-// registers past the 256 that two waves per SIMD allow (the handful of
-// spilled values cost less than the lost latency hiding).
+// Workgroups per CU the register allocation targets.  Volume variants: two
+// (they would otherwise take a few registers past the 256 that two waves per
+// SIMD allow; the handful of spilled values cost less than the lost latency
+// hiding, and a third wave per SIMD measured slower: c2 13.8 against 10.7
+// ms).  Thin-disk variants: three waves per SIMD in 168 registers
+// (c5 plain DP45 101 against 110 ms, records identical).
+template <bool VOL, bool POL>
+struct Fp64Occupancy {
+  static constexpr int kBlocks = (VOL || POL) ? 2 : 3;
+};
 template <int INTEG, bool VOL, bool MV, bool PATH, bool POL = false>
-__global__ void __launch_bounds__(kBlock, 2) k_trace_fp64(const DevParams* __restrict__ Pp) {
+__global__ void __launch_bounds__(kBlock, (Fp64Occupancy<VOL, POL>::kBlocks)) k_trace_fp64(
+    const DevParams* __restrict__ Pp) {
   const DevParams& P = *Pp;
   Lane L;
   L.gid = -1;

@@ -17,7 +17,10 @@
 //     in the compensated pair (y, cy), so consecutive island steps carry
 //     the FP64 state without rounding it to FP32;
 //   * FP32 radiative transfer with Kahan-compensated optical depth and
-//     intensity, in host-prescaled units (every intermediate in FP32 range).
+//     intensity, in host-prescaled units (every intermediate in FP32 range);
+//   * island steps gathered per wave (a wave runs the FP32 body or the FP64
+//     body in one iteration, never both), the FP32 steps' size control in
+//     FP32, and three waves per SIMD.
 // Replaces the reference's FP32 render kernel (geodesicTraceFp32,
 // src/cuda/kernels_fp32.cu:32-111) for the image-plane tracer.
 #include <hip/hip_runtime.h>
@@ -401,8 +404,32 @@ __device__ __forceinline__ float dp45_step_f(LaneF& L, const DevParams& P, float
   return err;
 }
 
+// DP45 step-size factor of the FP32 steps, in FP32: 0.9 err^(-1/5) with the
+// limits of dp45_factor (v_log_f32 and v_exp_f32 work in base 2).
+__device__ __forceinline__ float dp45_factor_f(float err, bool accepted, bool last_rejected) {
+  const float e = accepted ? fmaxf(err, 1e-10f) : err;
+  float fac = 0.9f * __builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf(e));
+  if (accepted) {
+    fac = fminf(fac, 5.0f);
+    if (last_rejected) fac = fminf(fac, 1.0f);
+  } else {
+    fac = fmaxf(fac, 0.2f);
+  }
+  return fac;
+}
+
+// Island steps are gathered per wave: a wave runs either the FP32 step body
+// (for its lanes that need no island) or the FP64 island body (for the
+// lanes that do), never both in one iteration.  It takes the island body once
+// kIslandGather lanes wait for it, or after kIslandWait iterations with any
+// lane waiting.  A lane's own sequence of steps does not change: waiting
+// iterations count neither as steps nor as attempts, so the records are
+// those of running every step in turn.
+constexpr int kIslandGather = 16;
+constexpr int kIslandWait = 12;
+
 template <int INTEG, bool VOL>
-__device__ __forceinline__ int step_f(LaneF& L, const DevParams& P) {
+__device__ __forceinline__ int step_f(LaneF& L, const DevParams& P, int& wait_it) {
   if (L.steps >= P.max_steps || L.attempts >= P.max_attempts) return BH_HIT_MAX_STEPS;
   if (INTEG == BH_INTEGRATOR_DP45) {
     float cap = static_cast<float>(P.h_max);
@@ -410,9 +437,17 @@ __device__ __forceinline__ int step_f(LaneF& L, const DevParams& P) {
       cap = static_cast<float>(P.h_vol);
     if (L.h > cap) L.h = cap;
   }
+  const bool isl = needs_island(L, P);
+  // wave-uniform choice of the step body (wait_it is read from the first
+  // active lane, so every lane of the wave sees the same count)
+  const int n_isl = __popcll(__ballot(isl));
+  const int w_it = __builtin_amdgcn_readfirstlane(wait_it);
+  const bool island_iter = n_isl > 0 && (n_isl >= kIslandGather || w_it >= kIslandWait);
+  wait_it = island_iter ? 0 : (n_isl > 0 ? w_it + 1 : 0);
+  if (isl != island_iter) return 0;  // this lane waits for the other body
   L.attempts++;
   bool hit = false;
-  if (needs_island(L, P)) {
+  if (isl) {
     double y0d[kN], f0d[kN], y1d[kN], f1d[kN], hd;
     if (!island_step<INTEG>(L, P, y0d, f0d, y1d, f1d, &hd)) return 0;  // rejected
     L.steps++;
@@ -436,11 +471,11 @@ __device__ __forceinline__ int step_f(LaneF& L, const DevParams& P) {
       h = L.h;
       const float err = dp45_step_f(L, P, h, y0, f0);
       if (err > 1.0f) {
-        L.h = h * static_cast<float>(dp45_factor(static_cast<double>(err), false, L.last_rej));
+        L.h = h * dp45_factor_f(err, false, L.last_rej);
         L.last_rej = true;
         return 0;
       }
-      L.h = h * static_cast<float>(dp45_factor(static_cast<double>(err), true, L.last_rej));
+      L.h = h * dp45_factor_f(err, true, L.last_rej);
       L.last_rej = false;
     }
     L.steps++;
@@ -451,12 +486,15 @@ __device__ __forceinline__ int step_f(LaneF& L, const DevParams& P) {
   return terminate_f(L, P);
 }
 
+// Three waves per SIMD (168 registers; the FP64 island body spills a few
+// values): c3 174 against 207 ms at two waves per SIMD.
 template <int INTEG, bool VOL>
-__global__ void __launch_bounds__(kBlock) k_trace_fp32(const DevParams* __restrict__ Pp) {
+__global__ void __launch_bounds__(kBlock, 3) k_trace_fp32(const DevParams* __restrict__ Pp) {
   const DevParams& P = *Pp;
   LaneF L;
   L.gid = -1;
   bool done = false;
+  int wait_it = 0;
   const long long n_queue = queue_length(P);
   for (;;) {
     const bool need = (L.gid < 0) && !done;
@@ -471,7 +509,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_fp32(const DevParams* __restri
     if (__ballot(!done) == 0ull) break;
     if (L.gid >= 0) {
       for (int b = 0; b < P.batch; ++b) {
-        const int st = step_f<INTEG, VOL>(L, P);
+        const int st = step_f<INTEG, VOL>(L, P, wait_it);
         if (st) {
           write_record_f(L, P, st);
           L.gid = -1;
