@@ -15,7 +15,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libbhk.so")
 
-SOURCES = ["trace.hip", "trace_fp32.hip", "surrogate.hip", "sur_infer.hip", "mino_disk.hip", "render_launch.hip", "bridge.hip", "grmhd.hip",
+SOURCES = ["trace.hip", "trace_fp32.hip", "surrogate.hip", "sur_infer.hip", "mino_disk.hip", "render_launch.hip", "bridge.hip", "grmhd.hip", "grmhd_stream.hip", "lut_manager.hip",
            "peaks.hip", "host_tables.cpp"]
 
 HIPCC_FLAGS = [

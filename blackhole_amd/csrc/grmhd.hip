@@ -225,6 +225,33 @@ bool grmhd_volume_known(unsigned long long h) {
   return g_vols.count(h) != 0;
 }
 
+// A volume like bh_grmhd_texture_create's, uploaded on stream s (host
+// texels should be pinned for the copy to be asynchronous); returns once the
+// upload has completed.  Released with bh_grmhd_texture_destroy.
+int grmhd_volume_create_on_stream(const float* host_rgba, int w, int h, int d, hipStream_t s,
+                                  unsigned long long* tex) {
+  const size_t cells = static_cast<size_t>(w) * h * d;
+  VolAlloc va{nullptr, nullptr};
+  if (hipMalloc(&va.data, cells * sizeof(float4)) != hipSuccess) return fail("grmhd volume: hipMalloc failed");
+  if (hipMalloc(&va.desc, sizeof(GrmhdTex)) != hipSuccess) {
+    (void)hipFree(va.data);
+    return fail("grmhd volume: hipMalloc failed");
+  }
+  const GrmhdTex dd{va.data, w, h, d, 0};
+  if (hipMemcpyAsync(va.data, host_rgba, cells * sizeof(float4), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(va.desc, &dd, sizeof(dd), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    (void)hipFree(va.data);
+    (void)hipFree(va.desc);
+    return fail("grmhd volume: upload failed");
+  }
+  const unsigned long long hd = reinterpret_cast<unsigned long long>(va.desc);
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_vols[hd] = va;
+  *tex = hd;
+  return 0;
+}
+
 __global__ void k_grmhd_sample(unsigned long long left, unsigned long long right, float alpha, float r_min,
                                float r_max, const float* __restrict__ xyz, int n, float4* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -157,3 +157,76 @@ def sample(left: int, right: int, alpha: float, r_min: float, r_max: float, xyz:
     check(_sigs().bh_grmhd_sample(left, right, alpha, r_min, r_max, xyz.ctypes.data_as(C.c_void_p), xyz.shape[0],
                                   out.ctypes.data_as(C.c_void_p)), "bh_grmhd_sample")
     return out
+
+
+class BH_GrmhdStreamStats(C.Structure):
+    _fields_ = [("hits", C.c_uint64), ("misses", C.c_uint64), ("resident_bytes", C.c_int64),
+                ("budget_bytes", C.c_int64), ("resident_frames", C.c_int32), ("queue_depth", C.c_int32),
+                ("current_frame", C.c_int32), ("frame_count", C.c_int32)]
+
+
+def _stream_sigs():
+    L = _sigs()
+    if getattr(L, "_grmhd_stream_sigs", False):
+        return L
+    L.bh_grmhd_streamer_create.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, C.c_int32]
+    L.bh_grmhd_streamer_create.restype = C.c_void_p
+    L.bh_grmhd_streamer_destroy.argtypes = [C.c_void_p]
+    L.bh_grmhd_streamer_destroy.restype = None
+    L.bh_grmhd_streamer_seek.argtypes = [C.c_void_p, C.c_int32]
+    L.bh_grmhd_streamer_get_frame.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_ulonglong)]
+    L.bh_grmhd_streamer_frame_pair.argtypes = [C.c_void_p, C.c_double, C.c_int32, C.POINTER(C.c_ulonglong),
+                                               C.POINTER(C.c_ulonglong), C.POINTER(C.c_float)]
+    L.bh_grmhd_streamer_stats.argtypes = [C.c_void_p, C.POINTER(BH_GrmhdStreamStats)]
+    for n in ("bh_grmhd_streamer_seek", "bh_grmhd_streamer_get_frame", "bh_grmhd_streamer_frame_pair",
+              "bh_grmhd_streamer_stats"):
+        getattr(L, n).restype = C.c_int
+    L._grmhd_stream_sigs = True
+    return L
+
+
+class Streamer:
+    """GRMHD time series streamed into device volumes (bh_grmhd_streamer_*,
+    the reference's GRMHDStreamer): an LRU frame cache in HBM under a byte
+    budget, a loader thread, and prefetch of the next frames on a seek."""
+
+    def __init__(self, json_path: str, bin_path: str, cache_bytes: int, prefetch: int = 3):
+        self.L = _stream_sigs()
+        self.h = self.L.bh_grmhd_streamer_create(json_path.encode(), bin_path.encode(), int(cache_bytes), prefetch)
+        if not self.h:
+            check(-1, "bh_grmhd_streamer_create")
+
+    def seek(self, frame: int) -> None:
+        check(self.L.bh_grmhd_streamer_seek(self.h, frame), "bh_grmhd_streamer_seek")
+
+    def get_frame(self, offset: int = 0, wait: bool = False) -> int:
+        """Device volume of the current (offset 0) or next (1) frame, 0 when not resident."""
+        t = C.c_ulonglong()
+        rc = self.L.bh_grmhd_streamer_get_frame(self.h, offset, 1 if wait else 0, C.byref(t))
+        if rc < 0:
+            check(rc, "bh_grmhd_streamer_get_frame")
+        return int(t.value)
+
+    def frame_pair(self, t: float, wait: bool = True):
+        """(left volume, right volume, alpha) around simulation time t."""
+        a, b, al = C.c_ulonglong(), C.c_ulonglong(), C.c_float()
+        rc = self.L.bh_grmhd_streamer_frame_pair(self.h, t, 1 if wait else 0, C.byref(a), C.byref(b), C.byref(al))
+        if rc < 0:
+            check(rc, "bh_grmhd_streamer_frame_pair")
+        return int(a.value), int(b.value), float(al.value)
+
+    def stats(self) -> dict:
+        s = BH_GrmhdStreamStats()
+        check(self.L.bh_grmhd_streamer_stats(self.h, C.byref(s)), "bh_grmhd_streamer_stats")
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def close(self) -> None:
+        if self.h:
+            self.L.bh_grmhd_streamer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

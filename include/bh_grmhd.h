@@ -85,6 +85,59 @@ int bh_grmhd_texture_destroy(unsigned long long tex);
 int bh_grmhd_sample(unsigned long long tex, unsigned long long tex_right, float alpha, float r_min,
                     float r_max, const float* xyz, int32_t n, float* out);
 
+/* ---- streaming of time series (src/grmhd_streaming.h:125-301) -----------
+ * The reference's GRMHDStreamer keeps recently used frames ("tiles" that are
+ * whole frames, grmhd_streaming.cpp:355-411) in a host LRU cache under a byte
+ * budget, loads missing ones on a background thread and prefetches the next
+ * three frames on every seek.  Here the cache holds frames as device volumes
+ * (bh_grmhd_texture_create's kind, ready for the render launch's grmhd
+ * slots), in HBM, under a device-memory budget: a frame is read from the
+ * binary into pinned host memory and uploaded on the streamer's own stream
+ * by the loader thread, so neither disk reads nor uploads block the caller.
+ * Eviction is least-recently-used and never takes the current frame or the
+ * next one; handles of those two stay valid until a later seek moves the
+ * current frame away (then they may be freed; launches using them must be
+ * synchronised first).  One host thread at a time per streamer. */
+typedef struct BH_GrmhdStreamer BH_GrmhdStreamer;
+
+typedef struct BH_GrmhdStreamStats {
+    uint64_t hits, misses;        /* frame look-ups that found / missed a resident frame */
+    int64_t  resident_bytes;      /* device bytes of resident frames               */
+    int64_t  budget_bytes;
+    int32_t  resident_frames, queue_depth;
+    int32_t  current_frame, frame_count;
+} BH_GrmhdStreamStats; /* 48 bytes */
+
+/* GRMHDStreamer(json, bin) + init(): parses the multi-frame metadata
+ * (bh_grmhd_timeseries_info), checks the binary holds every frame, and
+ * starts the loader thread.  cache_bytes: device budget (at least one frame);
+ * prefetch: frames after the current one loaded ahead on a seek (the
+ * reference uses 3).  NULL on failure (bh_last_error()). */
+BH_GrmhdStreamer* bh_grmhd_streamer_create(const char* json_path, const char* bin_path, int64_t cache_bytes,
+                                           int32_t prefetch);
+
+/* shutdown() and free: stops the loader, frees every resident frame; NULL is ignored. */
+void bh_grmhd_streamer_destroy(BH_GrmhdStreamer* s);
+
+/* seekFrame: the current frame (clamped to the series) and the prefetch. */
+int bh_grmhd_streamer_seek(BH_GrmhdStreamer* s, int32_t frame);
+
+/* getTile (offset 0: the current frame) / getAdjacentTile (offset 1: the
+ * next one): *tex = the resident frame's device volume and 0 returned, or
+ * *tex = 0 and 1 returned when it is not resident (for the current frame its
+ * load is queued; the adjacent frame is only looked up).  wait != 0 blocks
+ * until the frame's load has finished.  -1 on invalid arguments. */
+int bh_grmhd_streamer_get_frame(BH_GrmhdStreamer* s, int32_t offset, int32_t wait, unsigned long long* tex);
+
+/* The two frames around simulation time t (the rule of bh_grmhd_frame_pair)
+ * as device volumes and their blend weight: seeks to the left frame, then
+ * looks both up as bh_grmhd_streamer_get_frame does (right = 0 for a
+ * one-frame series).  Returns 0 when both are resident, 1 when one is not. */
+int bh_grmhd_streamer_frame_pair(BH_GrmhdStreamer* s, double t, int32_t wait, unsigned long long* left,
+                                 unsigned long long* right, float* alpha);
+
+int bh_grmhd_streamer_stats(const BH_GrmhdStreamer* s, BH_GrmhdStreamStats* out);
+
 #ifdef __cplusplus
 }
 #endif

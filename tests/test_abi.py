@@ -171,15 +171,17 @@ def test_struct_layouts_match_the_c_headers(tmp_path):
     from blackhole_amd import abi
     from blackhole_amd import surrogate as S
     from blackhole_amd import grmhd as G
+    from blackhole_amd import luts as LM
     structs = {
         "BH_Camera": abi.BH_Camera, "BH_TraceOptions": abi.BH_TraceOptions, "BH_Volume": abi.BH_Volume,
         "BH_VolumeSet": abi.BH_VolumeSet, "BH_TuneResult": abi.BH_TuneResult, "BH_RaySpec": abi.BH_RaySpec,
         "BH_SurrogateRenderParams": S.RenderParams, "BH_SurrogateConfig": S.SurrogateConfig,
         "BH_TorusParams": abi.BH_TorusParams, "BH_GrmhdPacked": G.BH_GrmhdPacked,
-        "BH_PolParams": abi.BH_PolParams,
+        "BH_PolParams": abi.BH_PolParams, "BH_LutImage": LM.BH_LutImage, "BH_LutTableDev": LM.BH_LutTableDev,
+        "BH_GrmhdStreamStats": G.BH_GrmhdStreamStats,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bhk_trace.h"', '#include "bhk_surrogate.h"',
-             '#include "bh_grmhd.h"',
+             '#include "bh_grmhd.h"', '#include "bh_lut_manager.h"',
              "int main(void) {"]
     for name, cls in structs.items():
         lines.append(f'  printf("{name} sizeof %zu\\n", sizeof({name}));')
