@@ -17,6 +17,7 @@
 #include <stdint.h>
 
 #include "bf16.hpp"
+#include "../../include/bhk_surrogate.h"
 
 namespace bhk {
 namespace mlp {
@@ -34,11 +35,20 @@ enum Epilogue { EPI_F32 = 0, EPI_BIAS_SILU = 1 };
 // lane l holds A[row = l % 16][k = 8 (l / 16) .. +7] and
 // B[col = l % 16][k = 8 (l / 16) .. +7]; the result lane l holds
 // C[row = 4 (l / 16) + i][col = l % 16], i = 0..3.
+//
+// Split K (gridDim.z > 1, EPI_F32 only): block z accumulates K-columns
+// [z K, (z + 1) K) of A and B (K is then the slice length) into the partial
+// C + z M ldc; k_sum_partials adds the partials in a fixed order.
 template <int EPI>
 __global__ void __launch_bounds__(256) k_gemm_nt(int M, int N, int K, const uint16_t* __restrict__ A,
                                                  int lda, const uint16_t* __restrict__ B, int ldb,
                                                  const float* __restrict__ bias, float* __restrict__ C,
                                                  int ldc, uint16_t* __restrict__ H, int ldh) {
+  if (gridDim.z > 1) {
+    A += static_cast<size_t>(blockIdx.z) * K;
+    B += static_cast<size_t>(blockIdx.z) * K;
+    C += static_cast<size_t>(blockIdx.z) * M * ldc;
+  }
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int m0 = blockIdx.y * 64 + (wave >> 1) * 32;
@@ -118,18 +128,38 @@ __global__ void k_silu_bwd(long long n, int cols, int ld_g, const float* __restr
   dZ[r * ld_d + c] = f2bf(g);
 }
 
-// db[c] = sum_r dZ[r][c] (BF16 input, FP32 output): one block per column group.
-__global__ void k_colsum_bf16(int rows, int cols, const uint16_t* __restrict__ dZ, int ld,
-                              float* __restrict__ db) {
-  const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int part = threadIdx.x >> 6;  // 4 partial sums per column
-  __shared__ float s[4][64];
+// Row chunk of the split reductions (weight gradients, layer-0 gradient).
+constexpr int kColRows = 512;
+
+// db[o] = sum_i dZt[o][i] for i < n (BF16 rows of the transposed dZ, row
+// stride ld, n and ld multiples of 8): one 256-thread block per row, 16-byte
+// loads, then a fixed-order tree in LDS.
+__global__ void k_rowsum_bf16(int n, const uint16_t* __restrict__ dZt, int ld, float* __restrict__ db) {
+  const uint16_t* row = dZt + static_cast<size_t>(blockIdx.x) * ld;
   float acc = 0.0f;
-  if (c < cols)
-    for (int r = part; r < rows; r += 4) acc += bf2f(dZ[static_cast<size_t>(r) * ld + c]);
-  s[part][threadIdx.x & 63] = acc;
+  for (int i = threadIdx.x * 8; i < n; i += 256 * 8) {
+    const uint4 v = *reinterpret_cast<const uint4*>(row + i);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc += bf2f(static_cast<uint16_t>(w[k] & 0xffffu)) + bf2f(static_cast<uint16_t>(w[k] >> 16));
+  }
+  __shared__ float s[256];
+  s[threadIdx.x] = acc;
   __syncthreads();
-  if (part == 0 && c < cols) db[c] = s[0][threadIdx.x] + s[1][threadIdx.x] + s[2][threadIdx.x] + s[3][threadIdx.x];
+  for (int st = 128; st > 0; st >>= 1) {
+    if (static_cast<int>(threadIdx.x) < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[blockIdx.x] = s[0];
+}
+
+// out[i] = sum_z part[z][i] for z = 0 .. nz - 1, in order.
+__global__ void k_sum_partials(long long n, int nz, const float* __restrict__ part, float* __restrict__ out) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float acc = 0.0f;
+  for (int z = 0; z < nz; ++z) acc += part[static_cast<size_t>(z) * n + i];
+  out[i] = acc;
 }
 
 __global__ void k_f32_to_bf16(long long n, const float* __restrict__ in, uint16_t* __restrict__ out) {
@@ -187,47 +217,74 @@ __global__ void k_adam(long long n, float* __restrict__ p, const float* __restri
 }
 
 // First layer in FP32 (the inputs are not quantised to BF16): Z0 = X W0^T + b0
-// over the n_feat live features, H1 = BF16(SiLU(Z0)).  One thread per
-// (row, output).  X [n][ldx], W0 [out][ldw], Z0 (optional) / H1 [n][out].
-__global__ void k_layer0_fwd(int n, int n_out, int n_feat, const float* __restrict__ X, int ldx,
-                             const float* __restrict__ W0, int ldw, const float* __restrict__ b0,
-                             float* __restrict__ Z0, uint16_t* __restrict__ H1) {
-  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= static_cast<long long>(n) * n_out) return;
-  const int r = static_cast<int>(i / n_out), o = static_cast<int>(i - static_cast<long long>(r) * n_out);
-  const float* x = X + static_cast<size_t>(r) * ldx;
-  const float* w = W0 + static_cast<size_t>(o) * ldw;
-  float z = b0[o];
-  for (int k = 0; k < n_feat; ++k) z = fmaf(w[k], x[k], z);
-  if (Z0) Z0[i] = z;
-  H1[i] = f2bf(silu(z));
+// over the n_feat live features, H1 = BF16(SiLU(Z0)).  Block (x, y) holds
+// the weights of outputs 256 y .. 256 y + 255 in LDS and runs rows
+// kL0Rows x .. + kL0Rows - 1, one output per thread (coalesced stores of Z0
+// and H1 rows).  X [n][ldx], W0 [out][ldw], Z0 (optional) / H1 [n][n_out];
+// grid (ceil(n / kL0Rows), ceil(n_out / 256)).
+constexpr int kL0Rows = 16;
+__global__ void __launch_bounds__(256) k_layer0_fwd(int n, int n_out, int n_feat, const float* __restrict__ X,
+                                                    int ldx, const float* __restrict__ W0, int ldw,
+                                                    const float* __restrict__ b0, float* __restrict__ Z0,
+                                                    uint16_t* __restrict__ H1) {
+  __shared__ float w[256][BH_SUR_FEATURES + 1];
+  __shared__ float x[kL0Rows][BH_SUR_FEATURES];
+  const int o0 = blockIdx.y * 256, o = o0 + threadIdx.x;
+  for (int i = threadIdx.x; i < 256 * BH_SUR_FEATURES; i += 256) {
+    const int oo = i / BH_SUR_FEATURES, k = i - oo * BH_SUR_FEATURES;
+    w[oo][k] = (o0 + oo < n_out && k < n_feat) ? W0[static_cast<size_t>(o0 + oo) * ldw + k] : 0.0f;
+  }
+  const int r0 = blockIdx.x * kL0Rows;
+  for (int i = threadIdx.x; i < kL0Rows * BH_SUR_FEATURES; i += 256) {
+    const int rr = i / BH_SUR_FEATURES, k = i - rr * BH_SUR_FEATURES;
+    x[rr][k] = (r0 + rr < n) ? X[static_cast<size_t>(r0 + rr) * ldx + k] : 0.0f;
+  }
+  __syncthreads();
+  if (o >= n_out) return;
+  const float b = b0[o];
+  for (int rr = 0; rr < kL0Rows && r0 + rr < n; ++rr) {
+    float z = b;
+    for (int k = 0; k < n_feat && k < BH_SUR_FEATURES; ++k) z = fmaf(w[threadIdx.x][k], x[rr][k], z);
+    const size_t i = static_cast<size_t>(r0 + rr) * n_out + o;
+    if (Z0) Z0[i] = z;
+    H1[i] = f2bf(silu(z));
+  }
 }
 
-// dW0[o][k] = sum_r dZ0[r][o] X[r][k] for k < n_feat (FP32; pad columns 0).
-// One block per output o; 256 threads stride over rows.
-__global__ void k_layer0_wgrad(int n, int n_out, int n_feat, const uint16_t* __restrict__ dZ0, int ldz,
-                               const float* __restrict__ X, int ldx, float* __restrict__ dW0, int ldw) {
-  const int o = blockIdx.x;
-  float acc[32];
+// dW0[o][k] = sum_r dZ0[r][o] X[r][k] for k < n_feat (FP32; pad columns 0),
+// in row chunks: block (x, y) covers outputs 64 x .. 64 x + 63 (one per lane;
+// dZ0 reads are coalesced, the X row is the same for every lane) and rows
+// [y kColRows, (y + 1) kColRows), four waves interleaving the rows, and
+// writes part[y][o][k] for k < ldw; k_sum_partials adds the chunks.
+__global__ void k_layer0_wgrad_part(int n, int n_out, int n_feat, const uint16_t* __restrict__ dZ0, int ldz,
+                                    const float* __restrict__ X, int ldx, float* __restrict__ part, int ldw) {
+  const int o = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int q = threadIdx.x >> 6;
+  const int r0 = blockIdx.y * kColRows;
+  const int r1 = min(n, r0 + kColRows);
+  float acc[BH_SUR_FEATURES];
 #pragma unroll
-  for (int k = 0; k < 32; ++k) acc[k] = 0.0f;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) {
-    const float g = bf2f(dZ0[static_cast<size_t>(r) * ldz + o]);
-    const float* x = X + static_cast<size_t>(r) * ldx;
+  for (int k = 0; k < BH_SUR_FEATURES; ++k) acc[k] = 0.0f;
+  if (o < n_out) {
+    for (int r = r0 + q; r < r1; r += 4) {
+      const float g = bf2f(dZ0[static_cast<size_t>(r) * ldz + o]);
+      const float* x = X + static_cast<size_t>(r) * ldx;
 #pragma unroll
-    for (int k = 0; k < 32; ++k)
-      if (k < n_feat) acc[k] = fmaf(g, x[k], acc[k]);
-  }
-  __shared__ float red[256];
-  for (int k = 0; k < ldw; ++k) {
-    red[threadIdx.x] = (k < 32) ? acc[k < 32 ? k : 0] : 0.0f;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
+      for (int k = 0; k < BH_SUR_FEATURES; ++k) acc[k] = fmaf(g, x[k], acc[k]);
     }
-    if (threadIdx.x == 0) dW0[static_cast<size_t>(o) * ldw + k] = (k < n_feat) ? red[0] : 0.0f;
-    __syncthreads();
+  }
+  __shared__ float s[4][64][BH_SUR_FEATURES + 1];
+#pragma unroll
+  for (int k = 0; k < BH_SUR_FEATURES; ++k) s[q][threadIdx.x & 63][k] = acc[k];
+  __syncthreads();
+  if (q == 0 && o < n_out) {
+    float* dst = part + (static_cast<size_t>(blockIdx.y) * n_out + o) * ldw;
+    for (int k = 0; k < ldw; ++k) {
+      const int l = threadIdx.x & 63;
+      dst[k] = (k < n_feat && k < BH_SUR_FEATURES)
+                   ? (s[0][l][k] + s[1][l][k]) + (s[2][l][k] + s[3][l][k])
+                   : 0.0f;
+    }
   }
 }
 

@@ -98,6 +98,7 @@ struct BH_Surrogate {
   uint16_t* d_t1 = nullptr;                  // transpose scratch [width][B]
   uint16_t* d_t2 = nullptr;                  // transpose scratch [width][B]
   float* d_loss = nullptr;                   // 2 floats
+  float* d_part = nullptr;                   // partial sums of the split reductions
   long long t = 0;                           // Adam step
   // Rendering scratch (sur_infer.hip), one buffer per (device, stream):
   // renders of one surrogate on different streams never share it.
@@ -166,8 +167,8 @@ void refresh_bf16(BH_Surrogate* s, hipStream_t st) {
 // runs in FP32 on the VALU (inputs unquantised), layers 1..L on BF16 MFMA.
 // keep_z: store the pre-activations (training).
 void forward(BH_Surrogate* s, int n, bool keep_z, hipStream_t st) {
-  hipLaunchKernelGGL(k_layer0_fwd, grid1(static_cast<long long>(n) * s->out_dim[0]), dim3(256), 0, st, n,
-                     s->out_dim[0], BH_SUR_FEATURES, s->d_x0, BH_SUR_IN_PAD, s->d_p + s->w_off[0],
+  hipLaunchKernelGGL(k_layer0_fwd, dim3((n + kL0Rows - 1) / kL0Rows, (s->out_dim[0] + 255) / 256), dim3(256), 0,
+                     st, n, s->out_dim[0], BH_SUR_FEATURES, s->d_x0, BH_SUR_IN_PAD, s->d_p + s->w_off[0],
                      BH_SUR_IN_PAD, s->d_p + s->b_off[0], keep_z ? s->d_z[0] : nullptr, s->d_x[1]);
   for (int l = 1; l < s->L; ++l) {
     const int N = s->out_dim[l], K = s->in_pad[l];
@@ -182,6 +183,7 @@ void forward(BH_Surrogate* s, int n, bool keep_z, hipStream_t st) {
                          BH_SUR_OUT, static_cast<uint16_t*>(nullptr), 0);
   }
 }
+
 
 // Loss + backward into d_g.
 void backward(BH_Surrogate* s, int n, const float* T, const float* Wt, hipStream_t st) {
@@ -203,12 +205,17 @@ void backward(BH_Surrogate* s, int n, const float* T, const float* Wt, hipStream
       dZ = s->d_dz;
       ldz = W;
     }
+    const int n_chunks = (n + kColRows - 1) / kColRows;
     if (l == 0) {
       // FP32 first layer: dW0 from the FP32 features
-      hipLaunchKernelGGL(k_layer0_wgrad, dim3(N_out), dim3(256), 0, st, n, N_out, BH_SUR_FEATURES, dZ, ldz,
-                         s->d_x0, BH_SUR_IN_PAD, s->d_g + s->w_off[0], K_in);
-      hipLaunchKernelGGL(k_colsum_bf16, dim3((N_out + 63) / 64), dim3(256), 0, st, n, N_out, dZ, ldz,
-                         s->d_g + s->b_off[0]);
+      hipLaunchKernelGGL(k_layer0_wgrad_part, dim3((N_out + 63) / 64, n_chunks), dim3(256), 0, st, n, N_out,
+                         BH_SUR_FEATURES, dZ, ldz, s->d_x0, BH_SUR_IN_PAD, s->d_part, K_in);
+      hipLaunchKernelGGL(k_sum_partials, grid1(static_cast<long long>(N_out) * K_in), dim3(256), 0, st,
+                         static_cast<long long>(N_out) * K_in, n_chunks, s->d_part, s->d_g + s->w_off[0]);
+      // bias gradient: row sums of dZ0^T
+      hipLaunchKernelGGL(k_transpose_bf16, dim3((N_out + 31) / 32, (n + 31) / 32), dim3(256), 0, st, n, N_out,
+                         dZ, ldz, s->d_t1, n);
+      hipLaunchKernelGGL(k_rowsum_bf16, dim3(N_out), dim3(256), 0, st, n, s->d_t1, n, s->d_g + s->b_off[0]);
       continue;
     }
     // grad W_l [N_out][K_in] = dZ^T X_l : A = dZ^T [N_out][n], B = X_l^T [K_in][n]
@@ -216,11 +223,19 @@ void backward(BH_Surrogate* s, int n, const float* T, const float* Wt, hipStream
                        dZ, ldz, s->d_t1, n);
     hipLaunchKernelGGL(k_transpose_bf16, dim3((K_in + 31) / 32, (n + 31) / 32), dim3(256), 0, st, n, K_in,
                        s->d_x[l], K_in, s->d_t2, n);
-    hipLaunchKernelGGL(k_gemm_nt<EPI_F32>, dim3((K_in + 63) / 64, (N_out + 63) / 64), dim3(256), 0, st, N_out,
-                       K_in, n, s->d_t1, n, s->d_t2, n, static_cast<const float*>(nullptr),
-                       s->d_g + s->w_off[l], K_in, static_cast<uint16_t*>(nullptr), 0);
-    hipLaunchKernelGGL(k_colsum_bf16, dim3((N_out + 63) / 64), dim3(256), 0, st, n, N_out, dZ, ldz,
-                       s->d_g + s->b_off[l]);
+    if (n % kColRows == 0 && n_chunks > 1) {
+      // split K = n over row chunks (the 256 x 256 result alone is only 16 tiles)
+      hipLaunchKernelGGL(k_gemm_nt<EPI_F32>, dim3((K_in + 63) / 64, (N_out + 63) / 64, n_chunks), dim3(256), 0, st,
+                         N_out, K_in, kColRows, s->d_t1, n, s->d_t2, n, static_cast<const float*>(nullptr),
+                         s->d_part, K_in, static_cast<uint16_t*>(nullptr), 0);
+      hipLaunchKernelGGL(k_sum_partials, grid1(static_cast<long long>(N_out) * K_in), dim3(256), 0, st,
+                         static_cast<long long>(N_out) * K_in, n_chunks, s->d_part, s->d_g + s->w_off[l]);
+    } else {
+      hipLaunchKernelGGL(k_gemm_nt<EPI_F32>, dim3((K_in + 63) / 64, (N_out + 63) / 64), dim3(256), 0, st, N_out,
+                         K_in, n, s->d_t1, n, s->d_t2, n, static_cast<const float*>(nullptr),
+                         s->d_g + s->w_off[l], K_in, static_cast<uint16_t*>(nullptr), 0);
+    }
+    hipLaunchKernelGGL(k_rowsum_bf16, dim3(N_out), dim3(256), 0, st, n, s->d_t1, n, s->d_g + s->b_off[l]);
     if (l > 0) {
       // dX_l [n][K_in] = dZ W_l : A = dZ [n][out_pad], B = W_l^T [K_in][out_pad]
       hipLaunchKernelGGL(k_gemm_nt<EPI_F32>, dim3((K_in + 63) / 64, (n + 63) / 64), dim3(256), 0, st, n,
@@ -279,7 +294,9 @@ BH_Surrogate* bh_surrogate_create(const BH_SurrogateConfig* cfg) {
   ok = ok && dalloc(&s->d_x0, static_cast<size_t>(B) * BH_SUR_IN_PAD) && dalloc(&s->d_y, static_cast<size_t>(B) * BH_SUR_OUT) && dalloc(&s->d_dx, static_cast<size_t>(B) * W) &&
        dalloc(&s->d_dz, static_cast<size_t>(B) * W) && dalloc(&s->d_dout, static_cast<size_t>(B) * 32) &&
        dalloc(&s->d_t1, static_cast<size_t>(B) * W) && dalloc(&s->d_t2, static_cast<size_t>(B) * W) &&
-       dalloc(&s->d_loss, 2);
+       dalloc(&s->d_loss, 2) &&
+       dalloc(&s->d_part, static_cast<size_t>((B + kColRows - 1) / kColRows) *
+                              std::max<size_t>(static_cast<size_t>(W) * W, static_cast<size_t>(W) * BH_SUR_IN_PAD));
   if (!ok) {
     fail("bh_surrogate_create: hipMalloc failed");
     bh_surrogate_destroy(s);
@@ -326,6 +343,7 @@ void bh_surrogate_destroy(BH_Surrogate* s) {
   for (auto p : s->d_x) f(p);
   for (auto p : s->d_z) f(p);
   f(s->d_x0); f(s->d_y); f(s->d_dx); f(s->d_dz); f(s->d_dout); f(s->d_t1); f(s->d_t2); f(s->d_loss);
+  f(s->d_part);
   for (auto& kv : s->render_bufs) f(kv.second.d);
   delete s;
 }

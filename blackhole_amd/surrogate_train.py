@@ -163,8 +163,33 @@ def load(path: str, max_batch: int = 16384) -> S.Surrogate:
     return sur
 
 
+def time_train_steps(batch: int = 16384, steps: int = 20, warmup: int = 3, seed: int = 5) -> dict:
+    """Wall time per training step (forward, loss, backward, Adam) of the
+    width-256, depth-6 network on a synthetic batch of `batch` rays with
+    random targets, inputs resident on the device."""
+    import torch
+    sur = S.Surrogate(width=256, depth=6, max_batch=batch, seed=seed)
+    rng = np.random.default_rng(seed)
+    rays = np.stack([rng.uniform(-20, 20, batch), rng.uniform(-20, 20, batch), rng.uniform(0, 0.998, batch),
+                     np.cos(np.radians(rng.uniform(5, 85, batch)))], 1).astype(np.float32)
+    feats = S.Surrogate.features(torch.from_numpy(rays).cuda())
+    T = torch.from_numpy(rng.normal(0, 1, (batch, S.OUT)).astype(np.float32)).cuda()
+    W = torch.ones((batch, S.OUT), dtype=torch.float32, device="cuda")
+    for _ in range(warmup):
+        sur.train_step(feats, T, W, 1e-4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = [sur.train_step(feats, T, W, 1e-4)[0] for _ in range(steps)]
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    return {"batch": batch, "steps": steps, "ms_per_step": ms, "rows_per_s": batch / ms * 1e3,
+            "loss_first": losses[0], "loss_last": losses[-1]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--time-steps", type=int, default=0,
+                    help="only time this many training steps on a synthetic batch and print them")
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--n-eval", type=int, default=1 << 18)
     ap.add_argument("--steps", type=int, default=40000)
@@ -174,6 +199,9 @@ def main() -> int:
     ap.add_argument("--out", default="")
     args = ap.parse_args()
     import torch  # noqa: F401
+    if args.time_steps > 0:
+        print("[time]", time_train_steps(max(args.batch, 64), args.time_steps), flush=True)
+        return 0
     t0 = time.time()
     feats, T, W, _ = build_dataset(args.n, args.seed)
     print(f"[data] {args.n} rays traced + targets in {time.time() - t0:.1f}s", flush=True)

@@ -29,13 +29,15 @@ def test_gemm_nt_mfma_layout_and_values():
         assert err <= 1e-4 * (ref.abs().max().item() + 1)
 
 
-def test_surrogate_gradients_match_fp64_oracle():
+@pytest.mark.parametrize("n,width", [(128, 64), (1536, 64)])
+def test_surrogate_gradients_match_fp64_oracle(n, width):
+    """n = 128: one row chunk; n = 1536: the weight and bias gradients split
+    over three 512-row chunks and summed in order."""
     import torch
     import mlp_oracle
     from blackhole_amd import surrogate as S
-    sur = S.Surrogate(width=64, depth=3, max_batch=128, seed=7)
+    sur = S.Surrogate(width=width, depth=3, max_batch=n, seed=7)
     rng = np.random.default_rng(5)
-    n = 128
     rays = np.stack([rng.uniform(-20, 20, n), rng.uniform(-20, 20, n), rng.uniform(0, 0.998, n),
                      np.cos(np.radians(rng.uniform(5, 85, n)))], axis=1).astype(np.float32)
     feats = S.Surrogate.features(torch.from_numpy(rays).cuda())
