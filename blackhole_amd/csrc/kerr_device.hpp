@@ -420,25 +420,111 @@ __device__ __forceinline__ double disk_intensity(double F, double g, double nu, 
 }
 
 // ---- volume ---------------------------------------------------------------
+//
+// The stepping above is reproducible bit for bit; the volume emission below
+// evaluates the same formulas as the FP64 oracle (trilinear fetch, fluid
+// redshift, thermal synchrotron j, Kirchhoff absorption) to a few units in
+// the last place per sample, not bit for bit: the divisions after the fluid
+// redshift share one reciprocal, the cell coordinates are scaled by
+// reciprocals, acos and the fractional powers of the synchrotron fit have
+// their own evaluations below, and expm1 of a small argument is its Taylor
+// polynomial.  The fluid redshift, a difference of large terms near the
+// light cylinder, keeps the oracle's operations.
+
+// Global address space for device loads known to hit device memory (the
+// compiler emits global rather than flat loads); the host pass only parses
+// device code, where the qualifier does not apply.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define BHK_GLOBAL __attribute__((address_space(1)))
+#else
+#define BHK_GLOBAL
+#endif
+
+struct VolDev {
+  const float4* cells;
+  int n_r, n_th, n_ph;
+  double r_min, r_max, r_span, sup_r_lo, sup_r_hi, sup_u_max, ell;
+  double s_r, s_th, s_ph, inv_nph;  // n_r / r_span, n_th / pi, n_ph / 2 pi, 1 / n_ph
+};
+
+// Radiative constants of a launch at observing frequency nu and length unit L:
+//   x_T = CT / (g Theta_e)            (h nu_f / k T_e, nu_f = nu / g)
+//   x_M = Cx / (g |B| Theta_e^2)      (nu_f / nu_s, nu_s = 1.5 nu_B Theta_e^2)
+//   j L = CjL n_e |B| I_M(x_M) / k2r  (k2r = K_2(1/Theta_e) / (2 Theta_e^2))
+//   B_nu(T_e) at nu_f = SB / (g^3 expm1(x_T)),  iSB = 1 / SB
+//   Faraday rate = FL n_e / g
+struct RadDev {
+  double CT, Cx, CjL, SB, iSB, FL;
+};
+
+// acos(u), |u| <= 1: acos = pi/2 - asin(u) for |u| <= 1/2, else
+// 2 asin(sqrt((1 - |u|) / 2)) reflected for u < 0, with asin(x) = x + x^3 P(x^2)
+// on |x| <= 1/2.  P is a degree-11 least-squares Chebyshev fit of the Taylor
+// series of (asin(x) - x) / x^3 in x^2 on [0, 1/4]; the result is within
+// 4.3e-16 relative of the correctly rounded acos over [-1, 1].
+__device__ __forceinline__ double acos_poly(double u) {
+  const double au = __builtin_fabs(u);
+  const bool big = au > 0.5;
+  const double w = (1.0 - au) * 0.5;
+  const double z = big ? w : u * u;
+  // sqrt(w) for w in [0, 1/4] (never subnormal unless 0): hardware reciprocal
+  // square root refined by two Newton-Raphson steps on the root and the half
+  // reciprocal (Goldschmidt form), within 1 ulp.
+  const double y0 = __builtin_amdgcn_rsq(w);
+  double sq = w * y0, hf = 0.5 * y0;
+  const double rr = __builtin_fma(-sq, hf, 0.5);
+  sq = __builtin_fma(sq, rr, sq);
+  hf = __builtin_fma(hf, rr, hf);
+  sq = __builtin_fma(__builtin_fma(-sq, sq, w), hf, sq);
+  const double x = big ? ((w > 0.0) ? sq : 0.0) : au;
+  double p = 0x1.ce1deb71b173dp-6;
+  p = __builtin_fma(p, z, -0x1.61a52073fdd05p-7);
+  p = __builtin_fma(p, z, 0x1.0712b4f81819bp-6);
+  p = __builtin_fma(p, z, 0x1.ff07c6f495617p-8);
+  p = __builtin_fma(p, z, 0x1.8527efdaa7650p-7);
+  p = __builtin_fma(p, z, 0x1.c8726e298da99p-7);
+  p = __builtin_fma(p, z, 0x1.1c59318cbcadfp-6);
+  p = __builtin_fma(p, z, 0x1.6e8b2ca05965dp-6);
+  p = __builtin_fma(p, z, 0x1.f1c71f7fec096p-6);
+  p = __builtin_fma(p, z, 0x1.6db6db689ba28p-5);
+  p = __builtin_fma(p, z, 0x1.3333333336569p-4);
+  p = __builtin_fma(p, z, 0x1.5555555555555p-3);
+  const double as = __builtin_fma(x * z, p, x);
+  if (big) return (u > 0.0) ? 2.0 * as : __builtin_fma(-2.0, as, kPi);
+  return 0.5 * kPi - ((u < 0.0) ? -as : as);
+}
+
+// expm1(x) for x >= 0: x (1 + x/2 + x^2/6 + x^3/24 + x^4/120) below 1e-3
+// (truncation x^6 / 720 < 1.4e-18 x), the library function above.
+__device__ __forceinline__ double expm1_pos(double x) {
+  if (x < 1e-3) {
+    const double q = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(x, 1.0 / 120.0, 1.0 / 24.0), x,
+                                                               1.0 / 6.0), x, 0.5), x, 1.0);
+    return x * q;
+  }
+  return expm1(x);
+}
 
 // Cell-centred trilinear sample of the cell fields {n_e, Theta_e, |B|,
 // K_2(1/Theta_e)/(2 Theta_e^2)} (one float4 per corner): r and theta clamp,
 // phi wraps (d_sample_grmhd, src/cuda/device_physics.cuh:1775-1829).
-__device__ __forceinline__ void volume_fetch(const float4* __restrict__ cells, int n_r, int n_th,
-                                             int n_ph, double r_min, double r_span, double r,
-                                             double u, double phi, double* out4) {
-  const double theta = acos(u);
-  const double xr = (r - r_min) / r_span * static_cast<double>(n_r) - 0.5;
-  const double xt = theta / kPi * static_cast<double>(n_th) - 0.5;
-  const double xp = phi / kTwoPi * static_cast<double>(n_ph) - 0.5;
+__device__ __forceinline__ void volume_fetch(const VolDev& v, double r, double u, double phi, double* out4) {
+  // the grid is in device global memory: global (not flat) loads
+  const BHK_GLOBAL float4* __restrict__ cells = (const BHK_GLOBAL float4*)(v.cells);
+  const int n_r = v.n_r, n_th = v.n_th, n_ph = v.n_ph;
+  const double xr = (r - v.r_min) * v.s_r - 0.5;
+  const double xt = acos_poly(u) * v.s_th - 0.5;
+  const double xp = phi * v.s_ph - 0.5;
   const double fr0 = floor(xr), ft0 = floor(xt), fp0 = floor(xp);
   const double fr = xr - fr0, ft = xt - ft0, fp = xp - fp0;
-  // 32-bit index arithmetic (grids are far below 2^31 cells); phi wraps by
-  // an exact floating-point modulo, so any finite phi maps into [0, n_ph).
+  // 32-bit index arithmetic (grids are far below 2^31 cells); phi wraps by a
+  // floating-point modulo whose quotient may be off by one, then an integer fix-up.
   int ir0 = static_cast<int>(fr0);
   int it0 = static_cast<int>(ft0);
   const double npd = static_cast<double>(n_ph);
-  int ip0 = static_cast<int>(fp0 - floor(fp0 / npd) * npd);
+  int ip0 = static_cast<int>(fp0 - floor(fp0 * v.inv_nph) * npd);
+  ip0 = (ip0 >= n_ph) ? ip0 - n_ph : ip0;
+  ip0 = (ip0 < 0) ? ip0 + n_ph : ip0;
   int ir1 = ir0 + 1, it1 = it0 + 1;
   ir0 = ir0 < 0 ? 0 : (ir0 > n_r - 1 ? n_r - 1 : ir0);
   ir1 = ir1 < 0 ? 0 : (ir1 > n_r - 1 ? n_r - 1 : ir1);
@@ -471,8 +557,14 @@ __device__ __forceinline__ void volume_fetch(const float4* __restrict__ cells, i
 #undef BHK_TRI
 }
 
-// Redshift of gas rotating with constant ell = -u_phi/u_t.
-__device__ __forceinline__ double fluid_g(double r, double u, double a, double xi, double ell) {
+// Redshift of gas rotating with constant ell = -u_phi/u_t as the numerator
+// sqrt(-(g_tt + 2 Omega g_tphi + Omega^2 g_phiphi)) and the denominator
+// 1 - Omega xi of g (false when g is not positive), Omega = -(ell g_tt +
+// g_tphi) / (ell g_tphi + g_phiphi).  The norm is a difference of large
+// terms near the light cylinder, so it keeps the oracle's order of
+// operations.
+__device__ __forceinline__ bool fluid_g_ratio(double r, double u, double a, double xi, double ell, double* S,
+                                              double* D) {
   const double s2 = __builtin_fma(-u, u, 1.0);
   const double a2 = a * a;
   const double sig = __builtin_fma(a2, u * u, r * r);
@@ -482,50 +574,59 @@ __device__ __forceinline__ double fluid_g(double r, double u, double a, double x
   const double gpp = __builtin_fma(a2 * two_r_sig, s2, __builtin_fma(r, r, a2)) * s2;
   const double Om = -__builtin_fma(ell, gtt, gtp) / __builtin_fma(ell, gtp, gpp);
   const double nrm = -__builtin_fma(Om * Om, gpp, __builtin_fma(2.0 * Om, gtp, gtt));
-  if (nrm <= 0.0) return 0.0;
   const double den = __builtin_fma(-Om, xi, 1.0);
-  if (den <= 0.0) return 0.0;
-  return __builtin_sqrt(nrm) / den;
+  if (!(nrm > 0.0) || !(den > 0.0)) return false;
+  *S = __builtin_sqrt(nrm);
+  *D = den;
+  return true;
 }
 
-// Thermal synchrotron j_nu (src/physics/rte_integrator.h:291-328) with
-// K_2(1/Theta) = 2 Theta^2 k2r.
-__device__ __forceinline__ double synch_j(double nu, double B, double ne, double th, double k2r) {
-  const double nuB = kECharge * B / (kTwoPi * kMElectron * kC);
-  const double nuS = 1.5 * nuB * th * th;
-  const double x = nu / nuS;
+// Redshift of gas rotating with constant ell (the value of fluid_g_ratio).
+__device__ __forceinline__ double fluid_g(double r, double u, double a, double xi, double ell) {
+  double S, D;
+  return fluid_g_ratio(r, u, a, xi, ell, &S, &D) ? S / D : 0.0;
+}
+
+// Thermal synchrotron fit I_M(x) = 4.0505 x^(-1/6) (1 + 0.40 x^(-1/4) +
+// 0.5316 x^(-1/2)) exp(-1.8899 x^(1/3)) (src/physics/rte_integrator.h:291-328).
+// For 1e-30 < x < 1e30 every power comes from y = x^(-1/12): an FP32
+// estimate 2^(-log2(x)/12) refined by two Newton steps of y^-12 = x
+// (y <- y (13 - x y^12) / 12, quadratic from ~1e-7), then x^(-1/6) = y^2,
+// x^(-1/4) = y^3, x^(-1/2) = y^6, x^(1/3) = x y^8.
+__device__ __forceinline__ double synch_im(double x) {
+  if (x > 1e-30 && x < 1e30) {
+    const float xf = static_cast<float>(x);
+    double y = static_cast<double>(__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(xf) * (-1.0f / 12.0f)));
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const double y2 = y * y;
+      const double y4 = y2 * y2;
+      const double y12 = y4 * y4 * y4;
+      y = y * (__builtin_fma(-x, y12, 13.0) * (1.0 / 12.0));
+    }
+    const double y2 = y * y;
+    const double y3 = y2 * y;
+    const double y6 = y3 * y3;
+    const double x13 = x * (y6 * y2);
+    return 4.0505 * y2 * __builtin_fma(0.5316, y6, __builtin_fma(0.40, y3, 1.0)) * exp(-1.8899 * x13);
+  }
   const double x13 = cbrt(x);
   const double xm12 = 1.0 / __builtin_sqrt(x);
   const double xm14 = __builtin_sqrt(xm12);
   const double xm16 = 1.0 / __builtin_sqrt(x13);
-  const double im = 4.0505 * xm16 * __builtin_fma(0.5316, xm12, __builtin_fma(0.40, xm14, 1.0)) *
-                    exp(-1.8899 * x13);
-  const double pref =
-      (1.7320508075688772 * kECharge * kECharge * kECharge / (kMElectron * kC * kC)) * ne * nuB;
-  return pref * im / (2.0 * k2r);
-}
-
-__device__ __forceinline__ double planck(double nu, double tempK) {
-  const double x = (kH / kKB) * nu / tempK;
-  if (x > 700.0) return 0.0;
-  return (2.0 * kH / (kC * kC)) * nu * nu * nu / expm1(x);
+  return 4.0505 * xm16 * __builtin_fma(0.5316, xm12, __builtin_fma(0.40, xm14, 1.0)) * exp(-1.8899 * x13);
 }
 
 // Emission / absorption per unit affine parameter in the observer's frame
 // at a volume point: nu_f = nu/g, alpha = j/B_nu(T_e) (Kirchhoff,
 // rte_integrator.h:348-357), a = alpha(nu_f) L / g, e = g^2 j(nu_f) L.
-struct VolDev {
-  const float4* cells;
-  int n_r, n_th, n_ph;
-  double r_min, r_max, r_span, sup_r_lo, sup_r_hi, sup_u_max, ell;
-};
-
 // rho (optional): the Faraday rotation rate per unit affine parameter,
 // faraday_per_ne n_e L / g (rad; zero outside the support).
-__device__ __forceinline__ bool volume_point(const VolDev& v, const RayC& c, double r, double u,
-                                             double phi, double nu_obs, double L, double* a,
-                                             double* e, double faraday_per_ne = 0.0,
-                                             double* rho = nullptr) {
+// The sample's remaining divisions share one reciprocal: with g = S / D
+// (fluid_g_ratio), T = Theta_e^2 |B| and Q = S D T k2r,
+//   g = S^2 T k2r / Q,  1/(g T) = D^2 k2r / Q,  1/g = T / (g T),  1/k2r = S D T / Q.
+__device__ __forceinline__ bool volume_point(const VolDev& v, const RayC& c, double r, double u, double phi,
+                                             const RadDev& K, double* a, double* e, double* rho = nullptr) {
   *a = 0.0;
   *e = 0.0;
   if (rho) *rho = 0.0;
@@ -533,18 +634,32 @@ __device__ __forceinline__ bool volume_point(const VolDev& v, const RayC& c, dou
         __builtin_fabs(u) <= v.sup_u_max))
     return false;
   double f4[4];
-  volume_fetch(v.cells, v.n_r, v.n_th, v.n_ph, v.r_min, v.r_span, r, u, phi, f4);
+  volume_fetch(v, r, u, phi, f4);
   const double ne = f4[0], th = f4[1], B = f4[2], k2r = f4[3];
   if (!(ne > 0.0 && th > 0.0 && B > 0.0 && k2r > 0.0)) return true;
-  const double g = fluid_g(r, u, c.a, c.xi, v.ell);
+  double S, D;
+  if (!fluid_g_ratio(r, u, c.a, c.xi, v.ell, &S, &D)) return true;
+  const double T = th * th * B;
+  const double SD = S * D;
+  const double rQ = 1.0 / (SD * (T * k2r));
+  const double inv_gT = (D * D) * k2r * rQ;  // 1 / (g Theta^2 |B|)
+  const double inv_g = inv_gT * T;
+  const double g = S * S * (T * k2r) * rQ;   // S / D
+  const double inv_k2r = SD * T * rQ;
   if (!(g > 0.0)) return true;
-  if (rho) *rho = faraday_per_ne * ne * L / g;
-  const double nu_f = nu_obs / g;
-  const double tempK = th * (kMElectron * kC * kC / kKB);
-  const double Bnu = planck(nu_f, tempK);
-  if (!(Bnu > 0.0)) return true;
-  const double jL = synch_j(nu_f, B, ne, th, k2r) * L;
-  *a = jL / (g * Bnu);
+  if (rho) *rho = K.FL * ne * inv_g;
+  const double xT = K.CT * (inv_gT * (th * B));  // CT / (g Theta)
+  if (xT > 700.0) return true;                   // B_nu underflows: no emission
+  const double jL = K.CjL * ne * B * synch_im(K.Cx * inv_gT) * inv_k2r;
+  if (xT > 600.0) {  // B_nu near underflow: the direct quotient
+    const double Bnu = K.SB * (inv_g * inv_g * inv_g) / expm1(xT);
+    if (!(Bnu > 0.0)) return true;
+    *a = jL / (g * Bnu);
+  } else {
+    // jL / B_nu scaled first: no intermediate smaller than the result (a
+    // product jL expm1(x_T) of a faint sample would go subnormal)
+    *a = (jL * K.iSB) * ((g * g) * expm1_pos(xT));
+  }
   *e = g * g * jL;
   return true;
 }
@@ -566,7 +681,14 @@ __device__ __forceinline__ void rt_segment(RtAcc& q, double h, double a1, double
     const double dl = 0.5 * h * (q.s0 + s1);
     if (asum > 0.0) {
       const double dtau = 0.5 * asum * dl;
-      const double om = -expm1(-dtau);
+      // 1 - e^-dtau: Taylor polynomial below 1e-3 (truncation dtau^6 / 720)
+      const double om = (dtau < 1e-3)
+                            ? dtau * __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(-dtau, 1.0 / 120.0,
+                                                                                               1.0 / 24.0),
+                                                                                -dtau, 1.0 / 6.0),
+                                                                 -dtau, 0.5),
+                                                   -dtau, 1.0)
+                            : -expm1(-dtau);
       q.I = __builtin_fma(q.Tr * (esum / asum), om, q.I);
       q.Tr = q.Tr * (1.0 - om);
       q.tau = q.tau + dtau;

@@ -88,8 +88,7 @@ __device__ __forceinline__ void volume_segment(Lane& L, const DevParams& P, cons
     v.cells = P.buckets[L.vb].cells;
     v.ell = P.buckets[L.vb].ell;
   }
-  if (volume_point(v, L.c, y[0], y[1], y[2], P.nu_obs, P.length_unit_cm, &a1, &e1, P.faraday_per_ne,
-                   POL ? &r1 : nullptr))
+  if (volume_point(v, L.c, y[0], y[1], y[2], P.rad, &a1, &e1, POL ? &r1 : nullptr))
     L.nsamp++;
   const double s1 = affine_rate(L.c, y[0], y[1]);
   if (POL) pol_segment(L.pol, L.rt, h, a1, e1, r1, s1, P.pol_c2, P.pol_s2, P.pol_frac);
@@ -485,6 +484,10 @@ VolDev vol_dev(const BH_Volume& v) {
   d.sup_r_hi = v.sup_r_hi;
   d.sup_u_max = v.sup_u_max;
   d.ell = v.ell;
+  d.s_r = static_cast<double>(v.n_r) / d.r_span;
+  d.s_th = static_cast<double>(v.n_th) / kPi;
+  d.s_ph = static_cast<double>(v.n_ph) / kTwoPi;
+  d.inv_nph = 1.0 / static_cast<double>(v.n_ph);
   return d;
 }
 
@@ -606,6 +609,11 @@ int build_launch(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts,
     P.CjL = static_cast<float>(1.7320508075688772 * kECharge * kECharge * kECharge / me_c2 * nuB_per_G * 0.5 *
                                o.length_unit_cm);
     P.Cd = static_cast<float>(o.disk_T_max > 0.0 ? kH * nu / (kKB * o.disk_T_max) : 1e30);
+    P.rad.CT = kH * nu / me_c2;
+    P.rad.Cx = nu / (1.5 * nuB_per_G);
+    P.rad.CjL = 1.7320508075688772 * kECharge * kECharge * kECharge / me_c2 * nuB_per_G * 0.5 * o.length_unit_cm;
+    P.rad.SB = 2.0 * kH * nu * nu * nu / (kC * kC);
+    P.rad.iSB = 1.0 / P.rad.SB;
   }
   im->key = key;
   im->blocks_per_cu = o.blocks_per_cu;
@@ -676,11 +684,14 @@ int launch_on_stream(const LaunchImage& im, void* stream, const char* who) {
 // Allocation-free launches (bh_trace_image_plane_noalloc): the queue head
 // and the launch block live in this file-scope device array, one copy per
 // GPU, held until the process ends.  Layout: [0, 256) the ray-queue head,
-// [256, kNoAllocBytes) the launch block.  Every call on a GPU shares its
+// [256, kNoAllocBytes) the launch block, sized for kNoAllocMaxCams cameras
+// without a volume set (about 64 KiB).  Every call on a GPU shares its
 // copy, so calls that overlap on one GPU (different streams or threads)
 // race; calls on one stream are ordered by the stream.
-constexpr size_t kNoAllocBytes = 64 * 1024;
+constexpr size_t kNoAllocMaxCams = 540;
 constexpr size_t kNoAllocHead = 256;
+constexpr size_t kNoAllocBytes =
+    (kNoAllocHead + sizeof(DevParams) + kNoAllocMaxCams * sizeof(CamDev) + 255) / 256 * 256;
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 std::mutex g_noalloc_mu;
@@ -716,11 +727,11 @@ unsigned char* noalloc_scratch(const char* who) {
 // no device allocation and no event.
 int launch_noalloc(const LaunchImage& im, void* stream, const char* who) {
   const size_t bytes = im.bytes();
-  if (bytes > kNoAllocBytes - kNoAllocHead)
-    return fail(std::string(who) + ": launch block of " + std::to_string(bytes) + " bytes exceeds the " +
+  if (bytes > kNoAllocBytes - kNoAllocHead || im.cams.size() > kNoAllocMaxCams)
+    return fail(std::string(who) + ": launch block of " + std::to_string(bytes) + " bytes (" +
+                std::to_string(im.cams.size()) + " cameras) exceeds the " +
                 std::to_string(kNoAllocBytes - kNoAllocHead) + "-byte scratch (at most " +
-                std::to_string((kNoAllocBytes - kNoAllocHead - sizeof(DevParams)) / sizeof(CamDev)) +
-                " cameras without a volume set)");
+                std::to_string(kNoAllocMaxCams) + " cameras without a volume set)");
   unsigned char* base = noalloc_scratch(who);
   if (!base) return -1;
   unsigned long long* d_head = reinterpret_cast<unsigned long long*>(base);
@@ -902,6 +913,7 @@ int bh_trace_image_plane_pol(const BH_Camera* cams, int n_cams, const BH_TraceOp
   im.P.pol_s2 = std::sin(2.0 * pp->evpa_b);
   im.P.pol_frac = pp->lin_pol_frac;
   im.P.faraday_per_ne = pp->faraday_per_ne;
+  im.P.rad.FL = pp->faraday_per_ne * im.P.length_unit_cm;
   im.key.pol = true;
   return launch_on_stream(im, stream, who);
 }

@@ -28,6 +28,7 @@ struct DevParams {
   double h_fixed, h_init, h_max, h_vol, rtol, atol;
   double r_escape, disk_r_out, nu_obs, disk_T_max, length_unit_cm, transmit_min;
   VolDev vol;
+  RadDev rad;
   const CamDev* cams;
   const double* luts;
   BH_PixelRecord* out;

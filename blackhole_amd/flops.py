@@ -3,7 +3,10 @@
 Counted by instantiating the oracle's templates with a counting scalar
 (oracle/flopcount.cpp: + - * / sqrt = 1, FMA = 2, transcendentals tallied
 apart); tests/test_oracle_pinning.py checks these constants against that
-program's output.  The GPU kernel performs exactly the same operations.
+program's output.  The GPU kernel performs the same stepping operations;
+per volume sample it evaluates the same formulas with fewer divisions and
+its own acos and fractional-power evaluations, which are not counted (the
+count is of the algorithm, not of an implementation).
 """
 RHS = 31
 RK4_STEP = 200          # 3 stage RHS + combine + RHS at the new point + projection

@@ -244,7 +244,7 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cameras,
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same arguments, kernels and records as bh_trace_image_plane;
- * the launch block and the ray-queue head live in a 64 KiB device array that
+ * the launch block and the ray-queue head live in a ~64 KiB device array that
  * the library holds once per GPU until the process ends (at most 540
  * cameras per call; larger launches return -1 with the limit in
  * bh_last_error()).  All calls on one GPU share that array: calls on one

@@ -29,11 +29,18 @@ def run_oracle(oracle, w: dict, ids=None, nthreads: int = 0) -> np.ndarray:
         vol=w.get("vol"), cells=cells.reshape(-1) if cells is not None else None, ids=ids, nthreads=nthreads)
 
 
-def rel(a, b):
+# Radiative values below this compare absolutely: a transfer sum this small
+# comes from samples whose emissivity is subnormal (j ~ 1e-310 at
+# nu / nu_s ~ 5e7), where neither the oracle nor the GPU keeps relative
+# precision (their results differ by up to ~1e-7 relative there).
+RADIATIVE_FLOOR = 1e-280
+
+
+def rel(a, b, floor: float = 1e-300):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     d = np.abs(a - b)
-    s = np.maximum(np.abs(b), 1e-300)
+    s = np.maximum(np.abs(b), floor)
     return np.where(d == 0, 0.0, d / s)
 
 
@@ -48,7 +55,7 @@ def compare(gpu: np.ndarray, ref: np.ndarray, tol: float = 1e-9) -> dict:
     }
     m = same_class
     for f in ("r_exit", "u_exit", "phi_exit", "intensity", "tau"):
-        r = rel(gpu[f][m], ref[f][m])
+        r = rel(gpu[f][m], ref[f][m], RADIATIVE_FLOOR if f in ("intensity", "tau") else 1e-300)
         out[f"max_rel_{f}"] = float(r.max()) if r.size else 0.0
     for f in ("drift_null", "drift_carter"):
         d = np.abs(gpu[f][m] - ref[f][m])

@@ -59,7 +59,9 @@ def test_pol_trace_invariants_512():
     assert np.array_equal(rec.view(np.uint8), base.view(np.uint8))
     I = sa[:, 0]
     pa = sa[:, 1] + 1j * sa[:, 2]
-    assert np.all(np.abs(pa) <= 0.7 * I * (1 + 1e-10))
+    # relative bound, plus an absolute floor for subnormal intensities (below
+    # 2.2e-308 a double has no relative precision left)
+    assert np.all(np.abs(pa) <= 0.7 * I * (1 + 1e-10) + 1e-300)
     _, sb = _gpu_pol(w, -0.5, 0.7, FAR)
     pb = sb[:, 1] + 1j * sb[:, 2]
     assert np.abs(pb - pa * np.exp(2j * (-0.8))).max() <= 1e-10 * I.max()
