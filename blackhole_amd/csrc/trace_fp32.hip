@@ -169,8 +169,17 @@ __device__ __forceinline__ float fluid_g_f(float r, float u, float a, float xi, 
   return __fdividef(sqrtT(nrm), den);
 }
 
+// expm1(x) for x >= 0 in FP32: x (1 + x/2 + x^2/6) below 1e-2 (truncation
+// x^3/24 < 4.2e-8 x), the library function above.
+__device__ __forceinline__ float expm1_pos_f(float x) {
+  return (x < 1e-2f) ? x * fmaT(fmaT(x, 1.0f / 6.0f, 0.5f), x, 1.0f) : expm1f(x);
+}
+
 // a (absorption) and e (emission) per affine length in the observer frame,
-// as volume_point in kerr_device.hpp, in FP32 with prescaled constants.
+// as volume_point in kerr_device.hpp, in FP32 with prescaled constants.  The
+// powers of the synchrotron fit come from the hardware base-2 logarithm and
+// exponential: x^(1/3) = 2^(l/3) and x^(-1/6), x^(-1/4), x^(-1/2) as
+// y^2, y^3, y^6 with y = 2^(-l/12), l = log2 x (a few ulp of FP32).
 __device__ __forceinline__ bool volume_point_f(const DevParams& P, const RayCT<float>& c, float r,
                                                float u, float phi, float* a, float* e) {
   const VolDev& v = P.vol;
@@ -187,15 +196,17 @@ __device__ __forceinline__ bool volume_point_f(const DevParams& P, const RayCT<f
   const float g = fluid_g_f(r, u, c.a, c.xi, static_cast<float>(v.ell));
   if (!(g > 0.0f)) return true;
   const float xT = __fdividef(P.CT, g * th);
-  const float em = expm1f(xT);
+  const float em = expm1_pos_f(xT);
   if (!(em > 0.0f) || xT > 80.0f) return true;
   const float g3 = g * g * g;
   const float Bnu = __fdividef(P.SB, g3 * em);
   const float x = __fdividef(P.Cx, g * B * th * th);
-  const float x13 = cbrtf(x);
-  const float xm12 = rsqrtf(x);
-  const float xm14 = sqrtT(xm12);
-  const float xm16 = rsqrtf(x13);
+  const float lx = __builtin_amdgcn_logf(x);
+  const float x13 = __builtin_amdgcn_exp2f(lx * (1.0f / 3.0f));
+  const float y = __builtin_amdgcn_exp2f(lx * (-1.0f / 12.0f));
+  const float xm16 = y * y;
+  const float xm14 = xm16 * y;
+  const float xm12 = xm14 * xm14;
   const float im = 4.0505f * xm16 * fmaT(0.5316f, xm12, fmaT(0.40f, xm14, 1.0f)) * __expf(-1.8899f * x13);
   const float jL = __fdividef(P.CjL * ne * B * im, k2r);
   *a = __fdividef(jL, g * Bnu);
@@ -210,7 +221,8 @@ __device__ __forceinline__ void rt_segment_f(LaneF& L, float h, float a1, float 
     const float dl = 0.5f * h * (L.s0 + s1);
     if (asum > 0.0f) {
       const float dtau = 0.5f * asum * dl;
-      const float om = -expm1f(-dtau);
+      // 1 - e^-dtau: dtau (1 - dtau/2 + dtau^2/6) below 1e-2
+      const float om = (dtau < 1e-2f) ? dtau * fmaT(fmaT(dtau, 1.0f / 6.0f, -0.5f), dtau, 1.0f) : -expm1f(-dtau);
       kahan_add(L.I, L.Ic, L.Tr * __fdividef(esum, asum) * om);
       L.Tr = L.Tr * (1.0f - om);
       kahan_add(L.tau, L.tauc, dtau);
