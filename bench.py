@@ -64,8 +64,11 @@ METRIC = "rays/s at ≤1e-3 rel intensity err: surrogate+fallback vs plain GPU D
 # --pmc WRITE_SIZE on this workload (bench.py --copy-calibration: FETCH_SIZE reads a 512 MiB
 # copy as 256 MiB, so fetches are scaled x 2; WRITE_SIZE reads it exactly).
 TRAFFIC_PER_RAY = {
-    "k_sur_render": (2 * 3530.0 + 8425000.0) * 1024 / 67108864,            # 128.6 B: the record + lists
-    "k_trace_fp64_dp45_disk": (2 * 314.0 + 12408500.0) * 1024 / 67108864,  # 189.3 B: record + crossing slots
+    "k_sur_render": (2 * 3270.0 + 8422133.0) * 1024 / 67108864,              # 128.6 B: the record + lists
+    # 226.2 B: record + crossing slots + the register spills of the three-waves-per-SIMD build
+    "k_trace_fp64_dp45_disk": (2 * 698113.0 + 13430814.0) * 1024 / 67108864,
+    # c2 at 1024 x 1024 (FP64): 407.2 B per ray, of which 128 B the record
+    "k_trace_fp64_dp45_volume_c2": (2 * 97762.0 + 221279.0) * 1024 / 1048576,
 }
 TRAFFIC_NOTE = "rocprofv3 FETCH_SIZE (x2, copy-calibrated) + WRITE_SIZE per dispatch, per ray x rays"
 FP64_VECTOR_PEAK_TFLOPS = 78.6
@@ -652,7 +655,10 @@ def bench_c2(args):
         },
         "roofline": {
             "bound": "valu", "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": None,
+            "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
+            "traffic": (TRAFFIC_PER_RAY["k_trace_fp64_dp45_volume_c2"] * sc.n_records
+                        if (size == 1024 and args.precision == "fp64") else None),
+            "traffic_source": TRAFFIC_NOTE,
             "kernel": f"k_trace_{args.precision}<DP45, volume>", "kernel_ms": k_ms, "flop_per_launch": f,
         },
         "frame_stats": {
