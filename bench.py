@@ -500,7 +500,8 @@ def bench_c4(args):
         res["parity_vs_oracle_on_sample"] = {
             "n": sum(p["n"] for p in pars), "class_mismatch": sum(p["class_mismatch"] for p in pars),
             "bitwise_exit": min(p["bitwise_exit"] for p in pars),
-            "max_rel_intensity": max(p["max_rel_intensity"] for p in pars)}
+            "max_rel_intensity": max(p["max_rel_intensity"] for p in pars),
+            "max_rel_tau": max(p["max_rel_tau"] for p in pars)}
     fr.close()
     return res
 
