@@ -57,6 +57,12 @@ def lib():
     ]
     L.bh_trace_image_plane_noalloc.argtypes = L.bh_trace_image_plane.argtypes
     L.bh_trace_image_plane_noalloc.restype = C.c_int
+    L.bh_symplectic_integrate.argtypes = [C.c_void_p, C.c_int64, _D, C.c_int32, _D, _D, C.c_void_p]
+    L.bh_symplectic_integrate.restype = C.c_int
+    L.bh_symplectic_hamiltonian.argtypes = [C.c_void_p, C.c_int64, _D, _D, C.c_void_p, C.c_void_p]
+    L.bh_symplectic_hamiltonian.restype = C.c_int
+    L.bh_symplectic_integrate_host.argtypes = [_PD, C.c_int64, _D, C.c_int32, _D, _D]
+    L.bh_symplectic_integrate_host.restype = C.c_int
     L.bh_trace_pixel_list.argtypes = [
         C.POINTER(abi.BH_Camera), C.c_int, C.POINTER(abi.BH_TraceOptions), C.c_void_p, C.c_int,
         C.POINTER(abi.BH_Volume), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]

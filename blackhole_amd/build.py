@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libbhk.so")
 
 SOURCES = ["trace.hip", "trace_fp32.hip", "surrogate.hip", "sur_infer.hip", "mino_disk.hip", "render_launch.hip", "bridge.hip", "grmhd.hip", "grmhd_stream.hip", "lut_manager.hip",
-           "peaks.hip", "host_tables.cpp"]
+           "symplectic.hip", "peaks.hip", "host_tables.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -44,6 +44,7 @@
 #include <utility>
 #include <vector>
 
+#include "bh_symplectic.h"
 #include "bhk_trace.h"
 
 namespace physics {
@@ -632,6 +633,29 @@ inline void integrateGeodesic(GeodesicState& s, double dLambda, int nSteps, cons
   if (nSteps <= 0) return;
   const double h = dLambda / static_cast<double>(nSteps);
   for (int i = 0; i < nSteps; ++i) yoshida4Step(s, h, k);
+}
+
+// integrateGeodesic for many states at once on the GPU (bh_symplectic.h):
+// the same composition, one state per lane.  False on a device error
+// (bh_last_error()); the states are then unchanged.
+inline bool integrateGeodesicBatch(std::vector<GeodesicState>& states, double dLambda, int nSteps,
+                                   const KerrParams& k) {
+  if (states.empty() || nSteps <= 0) return true;
+  std::vector<double> buf(states.size() * 8);
+  for (size_t i = 0; i < states.size(); ++i)
+    for (size_t j = 0; j < 4; ++j) {
+      buf[i * 8 + j] = states[i].q[j];
+      buf[i * 8 + 4 + j] = states[i].p[j];
+    }
+  if (bh_symplectic_integrate_host(buf.data(), static_cast<int64_t>(states.size()), dLambda, nSteps, k.a,
+                                   k.rS) != 0)
+    return false;
+  for (size_t i = 0; i < states.size(); ++i)
+    for (size_t j = 0; j < 4; ++j) {
+      states[i].q[j] = buf[i * 8 + j];
+      states[i].p[j] = buf[i * 8 + 4 + j];
+    }
+  return true;
 }
 
 // ---------------------------------------------------------------------------

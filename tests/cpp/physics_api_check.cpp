@@ -148,6 +148,27 @@ int main(int argc, char** argv) {
       for (int i = 0; i < 4; ++i) std::printf("%.17g ", st.p[i]);
       std::printf("%.17g\n", physics::kerrHamiltonian(st, k));
     }
+  } else if (!std::strcmp(what, "batch")) {
+    // rows a rS dLambda n t r theta phi p_t p_r p_theta p_phi (a, rS, dLambda, n from the first
+    // row) -> integrateGeodesicBatch on the GPU, one line per state (status -1 line on error)
+    double v[12];
+    std::vector<physics::GeodesicState> states;
+    double par[4] = {0.0, 0.0, 0.0, 0.0};
+    while (std::scanf("%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", v, v + 1, v + 2, v + 3, v + 4, v + 5,
+                      v + 6, v + 7, v + 8, v + 9, v + 10, v + 11) == 12) {
+      if (states.empty())
+        for (int i = 0; i < 4; ++i) par[i] = v[i];
+      states.push_back(physics::GeodesicState{{v[4], v[5], v[6], v[7]}, {v[8], v[9], v[10], v[11]}});
+    }
+    const physics::KerrParams k{par[0], par[1]};
+    if (!physics::integrateGeodesicBatch(states, par[2], static_cast<int>(par[3]), k)) {
+      std::printf("-1 %s\n", bh_last_error());
+      return 1;
+    }
+    for (const auto& st : states) {
+      for (int i = 0; i < 4; ++i) std::printf("%.17g ", st.q[i]);
+      for (int i = 0; i < 4; ++i) std::printf("%.17g%s", st.p[i], i == 3 ? "\n" : " ");
+    }
   } else if (!std::strcmp(what, "yoshida")) {
     // a rS h n t r theta phi p_t p_r p_theta p_phi -> the state after n steps and H
     double v[12];
