@@ -36,13 +36,14 @@ constexpr double kH = 2.0 * kPi * kHbar;
 constexpr double kProjKappa = 0.0625;
 
 // Scalar helpers: the FP64 instantiation uses correctly rounded operations
-// (parity with the oracle); the FP32 instantiation uses the fast hardware
-// forms (v_rcp_f32 / v_sqrt_f32 based), which the precision form's
-// tolerance admits.
+// (parity with the oracle); the FP32 instantiation uses the hardware
+// reciprocal and square root (v_rcp_f32, v_sqrt_f32: about 1 ulp), which the
+// precision form's tolerance admits.  (__fdividef compiles to the full
+// IEEE division sequence here, ten instructions.)
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double divT(double a, double b) { return a / b; }
-__device__ __forceinline__ float divT(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ float divT(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double sqrtT(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float sqrtT(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double fabsT(double x) { return __builtin_fabs(x); }

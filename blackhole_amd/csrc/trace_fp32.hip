@@ -51,7 +51,7 @@ __device__ __forceinline__ void kahan_add(float& sum, float& comp, float x) {
 }
 
 __device__ __forceinline__ float affine_rate_f(const RayCT<float>& c, float r, float u) {
-  return __fdividef(fmaT(c.a2, u * u, r * r), r);
+  return divT(fmaT(c.a2, u * u, r * r), r);
 }
 
 template <bool VOL>
@@ -111,14 +111,16 @@ __device__ __forceinline__ bool needs_island(const LaneF& L, const DevParams& P)
 __device__ __forceinline__ void volume_fetch_f(const VolDev& v, float r, float u, float phi,
                                                float* out4) {
   const float theta = acosf(u);
-  const float xr = (r - static_cast<float>(v.r_min)) * (static_cast<float>(v.n_r) / static_cast<float>(v.r_span)) - 0.5f;
-  const float xt = theta * (static_cast<float>(v.n_th) / static_cast<float>(kPi)) - 0.5f;
-  const float xp = phi * (static_cast<float>(v.n_ph) / static_cast<float>(kTwoPi)) - 0.5f;
+  const float xr = (r - static_cast<float>(v.r_min)) * static_cast<float>(v.s_r) - 0.5f;
+  const float xt = theta * static_cast<float>(v.s_th) - 0.5f;
+  const float xp = phi * static_cast<float>(v.s_ph) - 0.5f;
   const float fr0 = floorf(xr), ft0 = floorf(xt), fp0 = floorf(xp);
   const float fr = xr - fr0, ft = xt - ft0, fp = xp - fp0;
   int ir0 = static_cast<int>(fr0), it0 = static_cast<int>(ft0);
   const float npf = static_cast<float>(v.n_ph);
-  int ip0 = static_cast<int>(fp0 - floorf(fp0 / npf) * npf);
+  int ip0 = static_cast<int>(fp0 - floorf(fp0 * static_cast<float>(v.inv_nph)) * npf);
+  ip0 = (ip0 >= v.n_ph) ? ip0 - v.n_ph : ip0;  // the quotient may be off by one
+  ip0 = (ip0 < 0) ? ip0 + v.n_ph : ip0;
   int ir1 = ir0 + 1, it1 = it0 + 1;
   const int n_r = v.n_r, n_th = v.n_th, n_ph = v.n_ph;
   ir0 = ir0 < 0 ? 0 : (ir0 > n_r - 1 ? n_r - 1 : ir0);
@@ -127,7 +129,7 @@ __device__ __forceinline__ void volume_fetch_f(const VolDev& v, float r, float u
   it1 = it1 < 0 ? 0 : (it1 > n_th - 1 ? n_th - 1 : it1);
   ip0 = ip0 < 0 ? 0 : (ip0 > n_ph - 1 ? n_ph - 1 : ip0);
   const int ip1 = (ip0 + 1 == n_ph) ? 0 : ip0 + 1;
-  const float4* __restrict__ cells = v.cells;
+  const BHK_GLOBAL float4* __restrict__ cells = (const BHK_GLOBAL float4*)(v.cells);
   const unsigned b00 = static_cast<unsigned>((ip0 * n_th + it0) * n_r);
   const unsigned b01 = static_cast<unsigned>((ip0 * n_th + it1) * n_r);
   const unsigned b10 = static_cast<unsigned>((ip1 * n_th + it0) * n_r);
@@ -157,16 +159,16 @@ __device__ __forceinline__ float fluid_g_f(float r, float u, float a, float xi, 
   const float s2 = fmaT(-u, u, 1.0f);
   const float a2 = a * a;
   const float sig = fmaT(a2, u * u, r * r);
-  const float two_r_sig = __fdividef(2.0f * r, sig);
+  const float two_r_sig = divT(2.0f * r, sig);
   const float gtt = two_r_sig - 1.0f;
   const float gtp = -(a * two_r_sig * s2);
   const float gpp = fmaT(a2 * two_r_sig, s2, fmaT(r, r, a2)) * s2;
-  const float Om = -__fdividef(fmaT(ell, gtt, gtp), fmaT(ell, gtp, gpp));
+  const float Om = -divT(fmaT(ell, gtt, gtp), fmaT(ell, gtp, gpp));
   const float nrm = -fmaT(Om * Om, gpp, fmaT(2.0f * Om, gtp, gtt));
   if (nrm <= 0.0f) return 0.0f;
   const float den = fmaT(-Om, xi, 1.0f);
   if (den <= 0.0f) return 0.0f;
-  return __fdividef(sqrtT(nrm), den);
+  return divT(sqrtT(nrm), den);
 }
 
 // expm1(x) for x >= 0 in FP32: x (1 + x/2 + x^2/6) below 1e-2 (truncation
@@ -195,12 +197,12 @@ __device__ __forceinline__ bool volume_point_f(const DevParams& P, const RayCT<f
   if (!(ne > 0.0f && th > 0.0f && B > 0.0f && k2r > 0.0f)) return true;
   const float g = fluid_g_f(r, u, c.a, c.xi, static_cast<float>(v.ell));
   if (!(g > 0.0f)) return true;
-  const float xT = __fdividef(P.CT, g * th);
+  const float xT = divT(P.CT, g * th);
   const float em = expm1_pos_f(xT);
   if (!(em > 0.0f) || xT > 80.0f) return true;
   const float g3 = g * g * g;
-  const float Bnu = __fdividef(P.SB, g3 * em);
-  const float x = __fdividef(P.Cx, g * B * th * th);
+  const float Bnu = divT(P.SB, g3 * em);
+  const float x = P.Cx / (g * B * th * th);  // |B| th^2 can underflow: IEEE division
   const float lx = __builtin_amdgcn_logf(x);
   const float x13 = __builtin_amdgcn_exp2f(lx * (1.0f / 3.0f));
   const float y = __builtin_amdgcn_exp2f(lx * (-1.0f / 12.0f));
@@ -208,8 +210,8 @@ __device__ __forceinline__ bool volume_point_f(const DevParams& P, const RayCT<f
   const float xm14 = xm16 * y;
   const float xm12 = xm14 * xm14;
   const float im = 4.0505f * xm16 * fmaT(0.5316f, xm12, fmaT(0.40f, xm14, 1.0f)) * __expf(-1.8899f * x13);
-  const float jL = __fdividef(P.CjL * ne * B * im, k2r);
-  *a = __fdividef(jL, g * Bnu);
+  const float jL = (P.CjL * ne * B * im) / k2r;  // k2r of cold gas can be subnormal: IEEE division
+  *a = divT(jL, g * Bnu);
   *e = g * g * jL;
   return true;
 }
@@ -223,7 +225,7 @@ __device__ __forceinline__ void rt_segment_f(LaneF& L, float h, float a1, float 
       const float dtau = 0.5f * asum * dl;
       // 1 - e^-dtau: dtau (1 - dtau/2 + dtau^2/6) below 1e-2
       const float om = (dtau < 1e-2f) ? dtau * fmaT(fmaT(dtau, 1.0f / 6.0f, -0.5f), dtau, 1.0f) : -expm1f(-dtau);
-      kahan_add(L.I, L.Ic, L.Tr * __fdividef(esum, asum) * om);
+      kahan_add(L.I, L.Ic, L.Tr * (esum / asum) * om);  // asum can be subnormal: IEEE division
       L.Tr = L.Tr * (1.0f - om);
       kahan_add(L.tau, L.tauc, dtau);
     } else {
@@ -254,8 +256,8 @@ __device__ __forceinline__ bool crossing_f(LaneF& L, const DevParams& P, const T
     const double* lut = P.luts + static_cast<long long>(P.cams[L.cam].lut_index) * P.lut_n;
     const float F = static_cast<float>(lut_linear(lut, P.lut_n, static_cast<double>(L.r_isco), P.disk_r_out, rc));
     if (F > 0.0f && g > 0.0) {
-      const float x = __fdividef(P.Cd, static_cast<float>(g) * sqrtT(sqrtT(F)));
-      if (x < 80.0f) kahan_add(L.I, L.Ic, L.Tr * __fdividef(P.SB, expm1f(x)));
+      const float x = divT(P.Cd, static_cast<float>(g) * sqrtT(sqrtT(F)));
+      if (x < 80.0f) kahan_add(L.I, L.Ic, L.Tr * divT(P.SB, expm1f(x)));
     }
     return true;
   }
@@ -397,7 +399,7 @@ __device__ __forceinline__ float dp45_step_f(LaneF& L, const DevParams& P, float
                                   fmaT(float(e5), k5[i], fmaT(float(e4), k4[i], fmaT(float(e3), k3[i], float(e1) * k1[i])))));
     const float ay = fabsT(y[i]), ay5 = fabsT(y5[i]);
     const float sc = fmaT(P.rtol32, ay > ay5 ? ay : ay5, P.atol32);
-    const float qq = __fdividef(e, sc);
+    const float qq = divT(e, sc);
     acc = fmaT(qq, qq, acc);
   }
   const float err = sqrtT(acc * 0.2f);
