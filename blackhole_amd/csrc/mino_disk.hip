@@ -30,12 +30,22 @@ namespace bhk {
 int fail(const std::string& msg);
 
 __global__ void __launch_bounds__(256, 4) k_mino_disk(const MinoParams P) {
-  const long long gid = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  const long long q = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   bool fb = false;
-  if (gid < P.n_total) {
-    const int cam = find_camera(P.cams, P.n_cams, gid);
+  long long gid = q;  // the pixel this thread solves
+  if (q < P.n_total) {
+    const int cam = find_camera(P.cams, P.n_cams, q);
     const CamDev& k = P.cams[cam];
-    const long long local = gid - k.offset;
+    long long local = q - k.offset;
+    const long long npx = static_cast<long long>(k.w) * k.h;
+    if (local >= 0 && local < npx && npx < (1ll << 31)) {
+      // threads take the camera's pixels in 4 x 4 tiles (2-D neighbours in a wave)
+      long long px, py;
+      tile_order_pixel(static_cast<unsigned>(local), static_cast<unsigned>(k.w), static_cast<unsigned>(k.h), &px,
+                       &py);
+      local = py * k.w + px;
+      gid = k.offset + local;
+    }
     BH_PixelRecord* rec = P.out + gid;
     int* i4 = reinterpret_cast<int*>(rec);
     double2* d2 = reinterpret_cast<double2*>(rec);
