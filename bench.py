@@ -65,10 +65,10 @@ METRIC = "rays/s at ≤1e-3 rel intensity err: surrogate+fallback vs plain GPU D
 # copy as 256 MiB, so fetches are scaled x 2; WRITE_SIZE reads it exactly).
 TRAFFIC_PER_RAY = {
     "k_sur_render": (2 * 3270.0 + 8422133.0) * 1024 / 67108864,              # 128.6 B: the record + lists
-    # 226.2 B: record + crossing slots + the register spills of the three-waves-per-SIMD build
-    "k_trace_fp64_dp45_disk": (2 * 698113.0 + 13430814.0) * 1024 / 67108864,
-    # c2 at 1024 x 1024 (FP64): 407.2 B per ray, of which 128 B the record
-    "k_trace_fp64_dp45_volume_c2": (2 * 97762.0 + 221279.0) * 1024 / 1048576,
+    # 223.6 B: record + crossing slots + the register spills of the three-waves-per-SIMD build
+    "k_trace_fp64_dp45_disk": (2 * 670622.0 + 13315925.0) * 1024 / 67108864,
+    # c2 at 1024 x 1024 (FP64): 379.3 B per ray, of which 128 B the record
+    "k_trace_fp64_dp45_volume_c2": (2 * 83945.0 + 220529.0) * 1024 / 1048576,
 }
 TRAFFIC_NOTE = "rocprofv3 FETCH_SIZE (x2, copy-calibrated) + WRITE_SIZE per dispatch, per ray x rays"
 FP64_VECTOR_PEAK_TFLOPS = 78.6
