@@ -224,6 +224,34 @@ def frame_accuracy(sur_rec, plain_rec, cams):
     }
 
 
+def exit_direction_accuracy(rec, plain_rec):
+    """Escape directions of a pipeline's records against the plain FP64
+    records, on the device: over the rays both call escaped, the share whose
+    record carries a direction (finite u_exit, phi_exit: traced by the
+    integrator, or flagged BH_FLAG_EXIT_NETWORK, the network's -- the exact
+    solve computes none) and, over the network's, the angle between the two
+    directions (u = cos theta, phi) in radians."""
+    import torch
+    from blackhole_amd import abi
+    d_all = rec.view(torch.float64)
+    esc = (plain_rec[:, 112] == abi.HIT_ESCAPED) & (rec[:, 112] == abi.HIT_ESCAPED)
+    fin = esc & torch.isfinite(d_all[:, 2]) & torch.isfinite(d_all[:, 3])
+    net = fin & ((rec[:, 114] & abi.FLAG_EXIT_NETWORK) != 0)
+    n_esc, n_fin, n_net = int(esc.sum()), int(fin.sum()), int(net.sum())
+    out = {"escaped_rays": n_esc, "with_direction": n_fin, "with_direction_frac": n_fin / max(n_esc, 1),
+           "direction_from_network": n_net}
+    if n_net:
+        d, p = d_all[net], plain_rec.view(torch.float64)[net]
+        u1, u2 = d[:, 2].clamp(-1.0, 1.0), p[:, 2].clamp(-1.0, 1.0)
+        c = u1 * u2 + (1.0 - u1 * u1).sqrt() * (1.0 - u2 * u2).sqrt() * torch.cos(d[:, 3] - p[:, 3])
+        ang = torch.arccos(c.clamp(-1.0, 1.0))
+        samp = ang[:: max(1, ang.numel() // 4000000)]
+        out.update({"network_angle_mean_rad": float(ang.mean()), "network_angle_median_rad": float(samp.median()),
+                    "network_angle_p99_rad": float(torch.quantile(samp, 0.99)),
+                    "network_angle_max_rad": float(ang.max())})
+    return out
+
+
 def device_frame_flops(rec):
     """flops.frame_flops of a DP45 frame from the device records (uint8
     [n, 128]: steps, attempts and volume samples are int32 words 26, 27, 31;
@@ -281,11 +309,15 @@ def bench_c5(args):
     R.floor_I = floor_I
     sur_ms, sur_ev_ms = time_steps(R.run, args.steps, args.warmup)
     _, refine_render_ev_ms = time_steps(R.render, args.steps, 1)
+    # the network pass as it runs in the pipeline (refine mode: the decode leaves
+    # g and the intensity to the solve); the roofline's kernel time
+    _, net_ev_ms = time_steps(R._network, args.steps, 1)
     R.run()
     torch.cuda.synchronize()
     fb, au = R.counts_host()
     audit = R.audit_stats()
     acc = frame_accuracy(R.out, plain, w["cams"])
+    esc_dir = exit_direction_accuracy(R.out, plain)
     refined = (R.out[:, 114] & 0x10) != 0
     iters = R.out.view(torch.int32)[:, 26]
     it_sur = float(iters[refined].double().mean())
@@ -300,6 +332,7 @@ def bench_c5(args):
     torch.cuda.synchronize()
     fbm = M.fallback_count()
     accm = frame_accuracy(M.out, plain, w["cams"])
+    esc_dir_m = exit_direction_accuracy(M.out, plain)
     it_mino = float(M.out.view(torch.int32)[:, 26].double().mean())
     del M
     torch.cuda.empty_cache()
@@ -335,11 +368,13 @@ def bench_c5(args):
     }
     roof_plain["frac"] = roof_plain["achieved"] / roof_plain["peak"]
     roof_sur = {
-        "bound": "mfma", "achieved": f_sur / (render_ev_ms / 1000.0) / 1e12,
+        "bound": "mfma", "achieved": f_sur / (net_ev_ms / 1000.0) / 1e12,
         "peak": BF16_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s",
         "traffic": TRAFFIC_PER_RAY["k_sur_render"] * n if args.size == 2048 else None,
         "traffic_source": TRAFFIC_NOTE, "algorithmic_bytes": 128.0 * n,
-        "kernel": "k_sur_render (BF16 MFMA, FP32 accumulation)", "kernel_ms": render_ev_ms,
+        "kernel": "k_sur_render (BF16 MFMA, FP32 accumulation)", "kernel_ms": net_ev_ms,
+        "kernel_timing": "HIP events around the pipeline's network pass (refine mode) on its stream, "
+                         "with its two small set-up kernels",
         "flop_per_launch": f_sur, "flop_per_ray": f_ray,
     }
     roof_sur["frac"] = roof_sur["achieved"] / roof_sur["peak"]
@@ -373,9 +408,11 @@ def bench_c5(args):
         "surrogate_fallback": {
             "rays_per_s": sur_rate, "ms_per_step": sur_ms, "event_ms_per_step": sur_ev_ms,
             "speedup_vs_plain": sur_rate / plain_rate,
-            "render_kernel_ms": render_ev_ms, "render_plus_refine_ms": refine_render_ev_ms,
+            "render_kernel_ms": net_ev_ms, "render_plus_refine_ms": refine_render_ev_ms,
+            "network_alone_render_ms": render_ev_ms,
             "fallback_rays": fb, "fallback_frac": fb / n, "audit_rays": au, "newton_iters_per_ray": it_sur,
             "gate_pass": gate, "accuracy_vs_plain_fp64": summ(acc),
+            "escape_direction_vs_plain_fp64": esc_dir,
             "audit": {k: audit[k] for k in ("n", "mean_rel", "max_rel", "class_diff", "flux_weighted")},
             "roofline": roof_sur,
         },
@@ -389,6 +426,7 @@ def bench_c5(args):
             "solve_kernel_ms": mino_solve_ev_ms, "speedup_vs_plain": mino_rate / plain_rate,
             "fallback_rays": fbm, "newton_iters_per_ray": it_mino, "gate_pass": passes(accm),
             "accuracy_vs_plain_fp64": summ(accm),
+            "escape_direction_vs_plain_fp64": esc_dir_m,
         },
         "learned_part_net_effect": {
             "note": "the surrogate pipeline (network + exact refinement from its radii + fallback + audit) "
@@ -397,7 +435,9 @@ def bench_c5(args):
             "surrogate_pipeline_ms": sur_ms, "network_free_pipeline_ms": mino_ms,
             "ratio_network_free_ms_over_surrogate_ms": mino_ms / sur_ms,
             "newton_iters_from_network": it_sur, "newton_iters_from_hermite": it_mino,
-            "refine_from_network_ms": refine_render_ev_ms - render_ev_ms,
+            "refine_from_network_ms": refine_render_ev_ms - net_ev_ms,
+            "escape_directions": "the surrogate pipeline's escaped rays carry the network's escape direction "
+                                 "(flagged approximate); the network-free solve leaves it unknown (NaN)",
             "solve_from_hermite_ms": mino_solve_ev_ms,
         },
         "ceiling_at_full_bf16_peak": {

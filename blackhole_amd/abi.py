@@ -20,6 +20,7 @@ FLAG_FALLBACK = 0x02
 FLAG_AUDITED = 0x04
 FLAG_SURROGATE = 0x08
 FLAG_REFINED = 0x10
+FLAG_EXIT_NETWORK = 0x20
 
 
 class BH_Camera(C.Structure):

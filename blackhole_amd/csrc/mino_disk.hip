@@ -11,7 +11,10 @@
 // crossing radii, g from the exact Keplerian formula with the ray's xi (0
 // inside r_ISCO), the intensity of the crossing inside [r_ISCO, r_out] from
 // the camera's LUT by the integrator's formula, the hit class, the number of
-// crossings, steps = Newton iterations, flags BH_FLAG_REFINED.
+// crossings, steps = Newton iterations, flags BH_FLAG_REFINED.  The solve
+// does not produce the path's end direction: u_exit and phi_exit are NaN,
+// except in surrogate records the solve finds escaping, which keep the
+// network's escape direction and are flagged BH_FLAG_EXIT_NETWORK.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -84,17 +87,21 @@ __global__ void __launch_bounds__(256, 4) k_mino_disk(const MinoParams P) {
         } else {
           r_exit = (s.cls == mino::kHorizon) ? k.r_stop : P.r_escape;
         }
-        d2[0] = make_double2(I, r_exit);
         // the solve finds crossings and the class, not the path's end point:
-        // u_exit and phi_exit are marked unknown (NaN), also over a network's
-        // escape-direction outputs
-        d2[1] = make_double2(__builtin_nan(""), __builtin_nan(""));
+        // an escaping surrogate ray keeps the network's escape direction
+        // (flagged BH_FLAG_EXIT_NETWORK); otherwise u_exit and phi_exit are
+        // marked unknown (NaN)
+        const bool exit_net = P.refine && (flags_in & BH_FLAG_SURROGATE) && s.cls == mino::kEscaped;
+        const double2 dir =
+            exit_net ? d2[1] : make_double2(__builtin_nan(""), __builtin_nan(""));
+        d2[0] = make_double2(I, r_exit);
+        d2[1] = dir;
         d2[2] = make_double2(s.r[0], s.r[1]);
         d2[3] = make_double2(s.r[2], g[0]);
         d2[4] = make_double2(g[1], g[2]);
         d2[5] = make_double2(0.0, 0.0);
         *reinterpret_cast<int4*>(i4 + 24) = make_int4(0, 0, s.iters, 0);  // drift_carter, steps, attempts
-        const unsigned flags = (flags_in & BH_FLAG_SURROGATE) | BH_FLAG_REFINED;
+        const unsigned flags = (flags_in & BH_FLAG_SURROGATE) | BH_FLAG_REFINED | (exit_net ? BH_FLAG_EXIT_NETWORK : 0u);
         tail.x = static_cast<int>(static_cast<unsigned>(s.cls) | (static_cast<unsigned>(min(s.n, 255)) << 8) |
                                   (flags << 16));
         *reinterpret_cast<int4*>(i4 + 28) = tail;

@@ -404,8 +404,13 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
       r_exit = k.r_stop;
     } else {
       r_exit = P.r_escape;
+    }
+    // the escape direction: for the network's escaped rays, and with the solve
+    // to follow for every ray (the solve keeps it where it finds the ray
+    // escaping; it computes crossings and the class, not the path's end)
+    if (cls == BH_HIT_ESCAPED || P.refine) {
       u_exit = O[BH_SUR_ESC_U];
-      phi_exit = atan2(static_cast<double>(O[BH_SUR_ESC_SINPHI]), static_cast<double>(O[BH_SUR_ESC_COSPHI]));
+      phi_exit = atan2f(O[BH_SUR_ESC_SINPHI], O[BH_SUR_ESC_COSPHI]);
     }
     BH_PixelRecord* rec = P.out + gid;
     double2* d2 = reinterpret_cast<double2*>(rec);

@@ -134,7 +134,10 @@ typedef struct BH_SurrogateRenderParams {
  * (features, 7 layers and the decode in one launch; width 256 only): the
  * thin-disk record (hit class, first crossings, g from the exact Keplerian
  * formula at the predicted radius, intensity from the disk LUT, flags
- * BH_FLAG_SURROGATE and, for rays that must be traced, BH_FLAG_FALLBACK).
+ * BH_FLAG_SURROGATE and, for rays that must be traced, BH_FLAG_FALLBACK;
+ * u_exit, phi_exit the network's escape direction for its escaped rays and,
+ * with refine set, for every ray -- the solve keeps it, flagged
+ * BH_FLAG_EXIT_NETWORK, where it finds the ray escaping).
  * Ids of fallback rays are appended to fallback_ids (count in device memory,
  * zeroed by the call), ids of a deterministic 1-in-audit_every sample of the
  * other rays to audit_ids; both lists are in the form bh_trace_pixel_list

@@ -72,6 +72,9 @@ enum BH_Precision {
 #define BH_FLAG_AUDITED       0x04u /* ray traced for the surrogate audit        */
 #define BH_FLAG_SURROGATE     0x08u /* record produced by the surrogate          */
 #define BH_FLAG_REFINED       0x10u /* crossings solved by the Mino-time solver   */
+#define BH_FLAG_EXIT_NETWORK  0x20u /* u_exit, phi_exit are the surrogate
+                                       network's escape direction (approximate:
+                                       neither solved nor integrated)          */
 
 /* ---------------------------------------------------------------------- */
 /* Inputs                                                                  */
@@ -385,8 +388,9 @@ int bh_trace_image_plane_pol(const BH_Camera* cams, int n_cams, const BH_TraceOp
  * iterations, flags BH_FLAG_REFINED).  Same cameras, options and LUTs as
  * bh_trace_image_plane with disk_enabled = 1 and volume_enabled = 0; the
  * path's end point direction (u_exit, phi_exit) is not computed and is
- * written as NaN (also in records this solver refines, over a surrogate's
- * escape-direction outputs).  Rays the solver
+ * written as NaN (in records this solver refines, an escaping ray keeps the
+ * surrogate network's escape direction instead, flagged
+ * BH_FLAG_EXIT_NETWORK).  Rays the solver
  * does not resolve (within about 1e-3 M of the critical curve) get
  * BH_FLAG_FALLBACK and their ids in fallback_ids (count in device memory,
  * zeroed by the call) for bh_trace_pixel_list.  No counterpart in the

@@ -226,3 +226,14 @@ def test_frame_and_volume_set_validation_without_gpu():
     vs2 = abi.BH_VolumeSet(vols2, 2, 0, cv2)
     assert lib().bh_trace_image_plane_multi(cams, 2, C.byref(opts), None, 0, C.byref(vs2), C.c_void_p(1), None) == -1
     assert b"volume index" in lib().bh_last_error()
+
+
+def test_record_flag_bits_match_the_header():
+    """abi.FLAG_* are the BH_FLAG_* macros of bhk_trace.h, distinct single bits."""
+    txt = open(os.path.join(ROOT, "include", "bhk_trace.h")).read()
+    macros = {m.group(1): int(m.group(2), 16) for m in re.finditer(r"#define BH_FLAG_(\w+)\s+0x([0-9a-fA-F]+)u", txt)}
+    assert len(macros) >= 6
+    for name, v in macros.items():
+        assert getattr(abi, "FLAG_" + name) == v, name
+        assert v & (v - 1) == 0, name
+    assert len(set(macros.values())) == len(macros)

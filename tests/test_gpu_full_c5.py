@@ -7,7 +7,9 @@ Size-independent properties only (the oracle takes minutes per frame):
 per frame, mean relative intensity error over pixels with
 I_ref > 1e-6 max(I_ref) at most 1e-6, hit classes differing on at most 1e-6
 of the pixels, shadow (horizon-class) area identical to within 2 pixels;
-the audit sees no class difference and a mean error below 1e-6.  The plain
+the audit sees no class difference and a mean error below 1e-6; every ray
+both images call escaped carries an escape direction, the network's (flagged
+approximate) within 0.05 rad on average.  The plain
 images themselves equal the oracle bit for bit on sampled pixels
 (tests/test_gpu_parity.py, bench.py's sample)."""
 from __future__ import annotations
@@ -63,3 +65,16 @@ def test_fast_paths_match_the_integrator_at_full_size():
         assert rel <= 1e-6 and cls <= 1e-6 * n and abs(shadow) <= 2, (k, rel, cls, shadow)
     st = R.audit_stats()
     assert au > 0.005 * sc.n_records and st["class_diff"] == 0 and st["mean_rel"] <= 1e-6
+    # escape directions: every ray both images call escaped carries one (the
+    # integrator's for fallback rays, the network's, flagged, for the others)
+    from blackhole_amd import abi
+    d, p = R.out.view(torch.float64), plain.view(torch.float64)
+    esc = (R.out[:, 112] == abi.HIT_ESCAPED) & (plain[:, 112] == abi.HIT_ESCAPED)
+    assert bool((torch.isfinite(d[esc, 2]) & torch.isfinite(d[esc, 3])).all())
+    net = esc & ((R.out[:, 114] & abi.FLAG_EXIT_NETWORK) != 0)
+    u1, u2 = d[net, 2].clamp(-1.0, 1.0), p[net, 2].clamp(-1.0, 1.0)
+    ang = torch.arccos((u1 * u2 + (1 - u1 * u1).sqrt() * (1 - u2 * u2).sqrt()
+                        * torch.cos(d[net, 3] - p[net, 3])).clamp(-1.0, 1.0))
+    print("[full c5] network escape directions:", int(net.sum()), "of", int(esc.sum()),
+          "escaped rays, mean angle", float(ang.mean()), flush=True)
+    assert int(net.sum()) > 0.9 * int(esc.sum()) and float(ang.mean()) <= 0.05

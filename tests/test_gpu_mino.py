@@ -38,6 +38,14 @@ def _check_disk_records(got, ref, solved):
     return float(rel_I.mean())
 
 
+def exit_angle(got, ref, sel):
+    """Angle (rad) between two records' exit directions (u = cos theta, phi)."""
+    u1, u2 = np.clip(got["u_exit"][sel], -1.0, 1.0), np.clip(ref["u_exit"][sel], -1.0, 1.0)
+    s1, s2 = np.sqrt(1.0 - u1 * u1), np.sqrt(1.0 - u2 * u2)
+    c = u1 * u2 + s1 * s2 * np.cos(got["phi_exit"][sel] - ref["phi_exit"][sel])
+    return np.arccos(np.clip(c, -1.0, 1.0))
+
+
 def test_mino_disk_matches_oracle_and_falls_back_exactly():
     import torch
     w = workloads.c5(40)  # the 16 frames of the surrogate comparison at 40 x 40
@@ -54,6 +62,7 @@ def test_mino_disk_matches_oracle_and_falls_back_exactly():
     assert ((got["flags"][~flagged] & abi.FLAG_REFINED) != 0).all()
     # the solver does not compute the path's end direction: marked unknown
     assert np.isnan(got["u_exit"][~flagged]).all() and np.isnan(got["phi_exit"][~flagged]).all()
+    assert ((got["flags"] & abi.FLAG_EXIT_NETWORK) == 0).all()
     _check_disk_records(got, ref, ~flagged)
     R.fallback()
     torch.cuda.synchronize()
@@ -86,6 +95,17 @@ def test_refined_surrogate_render_matches_oracle():
     _check_disk_records(got, ref, np.ones(len(got), bool))
     refined = (got["flags"] & abi.FLAG_REFINED) != 0
     assert refined.mean() > 0.9
+    # escape direction: the solve keeps the network's for the rays it finds
+    # escaping (flagged approximate), and marks it unknown for the others
+    esc = refined & (got["hit_class"] == abi.HIT_ESCAPED)
+    net_dir = (got["flags"] & abi.FLAG_EXIT_NETWORK) != 0
+    assert esc.sum() > 1000 and np.array_equal(net_dir, esc)
+    assert np.isfinite(got["u_exit"][esc]).all() and np.isfinite(got["phi_exit"][esc]).all()
+    assert np.isnan(got["u_exit"][refined & ~esc]).all() and np.isnan(got["phi_exit"][refined & ~esc]).all()
+    ang = exit_angle(got, ref, esc)
+    print("[refine] network escape direction vs oracle: mean", ang.mean(), "p99", np.percentile(ang, 99),
+          flush=True)
+    assert ang.mean() <= 0.05 and np.percentile(ang, 99) <= 0.3
     fresh = tracer.records_to_numpy(M.out)
     both = refined & ((fresh["flags"] & abi.FLAG_REFINED) != 0)
     it_guess, it_fresh = got["steps"][both].mean(), fresh["steps"][both].mean()

@@ -142,6 +142,15 @@ def test_fused_render_matches_layerwise_forward_oracle_and_decode(depth):
     sc0 = np.abs(y_ref[:, S.LOGR1]).max() + 1e-3
     assert has.sum() > 0
     assert np.abs(lnr - y_ref[has, S.LOGR1]).max() <= 0.05 * sc0
+    # escape direction of escaped records: u and atan2(sin phi, cos phi) outputs
+    esc = r["hit_class"] == abi.HIT_ESCAPED
+    assert esc.sum() > 0
+    su = np.abs(y_ref[:, S.ESC_U]).max() + 1e-3
+    assert np.abs(r["u_exit"][esc] - y_ref[esc, S.ESC_U]).max() <= 0.05 * su
+    phi_ref = np.arctan2(y_ref[esc, S.ESC_SINPHI], y_ref[esc, S.ESC_COSPHI])
+    big = np.hypot(y_ref[esc, S.ESC_SINPHI], y_ref[esc, S.ESC_COSPHI]) > 0.2  # angle defined
+    dphi = np.angle(np.exp(1j * (r["phi_exit"][esc] - phi_ref)))
+    assert np.abs(dphi[big]).max() <= 0.1, np.abs(dphi[big]).max()
     # decode: intensity of disk records from the oracle's disk formulas
     L = OL.lib()
     for k, cam in enumerate(cams):
