@@ -227,10 +227,13 @@ def frame_accuracy(sur_rec, plain_rec, cams):
 def exit_direction_accuracy(rec, plain_rec):
     """Escape directions of a pipeline's records against the plain FP64
     records, on the device: over the rays both call escaped, the share whose
-    record carries a direction (finite u_exit, phi_exit: traced by the
-    integrator, or flagged BH_FLAG_EXIT_NETWORK, the network's -- the exact
-    solve computes none) and, over the network's, the angle between the two
-    directions (u = cos theta, phi) in radians."""
+    record carries a direction (finite u_exit, phi_exit: the exact solve's at
+    r_escape, the integrator's for fallback rays, or -- flagged
+    BH_FLAG_EXIT_NETWORK -- the network's where the solve gives none) and the
+    angle between the two directions (u = cos theta, phi) in radians.  The
+    plain integrator records its direction where its last step ends (r_exit
+    200 - 213 M), so the angle includes the turn between r_escape and that
+    radius."""
     import torch
     from blackhole_amd import abi
     d_all = rec.view(torch.float64)
@@ -240,15 +243,20 @@ def exit_direction_accuracy(rec, plain_rec):
     n_esc, n_fin, n_net = int(esc.sum()), int(fin.sum()), int(net.sum())
     out = {"escaped_rays": n_esc, "with_direction": n_fin, "with_direction_frac": n_fin / max(n_esc, 1),
            "direction_from_network": n_net}
-    if n_net:
-        d, p = d_all[net], plain_rec.view(torch.float64)[net]
+
+    def angles(sel):
+        d, p = d_all[sel], plain_rec.view(torch.float64)[sel]
         u1, u2 = d[:, 2].clamp(-1.0, 1.0), p[:, 2].clamp(-1.0, 1.0)
         c = u1 * u2 + (1.0 - u1 * u1).sqrt() * (1.0 - u2 * u2).sqrt() * torch.cos(d[:, 3] - p[:, 3])
         ang = torch.arccos(c.clamp(-1.0, 1.0))
         samp = ang[:: max(1, ang.numel() // 4000000)]
-        out.update({"network_angle_mean_rad": float(ang.mean()), "network_angle_median_rad": float(samp.median()),
-                    "network_angle_p99_rad": float(torch.quantile(samp, 0.99)),
-                    "network_angle_max_rad": float(ang.max())})
+        return {"mean_rad": float(ang.mean()), "median_rad": float(samp.median()),
+                "p99_rad": float(torch.quantile(samp, 0.99)), "max_rad": float(ang.max())}
+
+    if n_fin:
+        out["angle_to_plain"] = angles(fin)
+    if n_net:
+        out["angle_to_plain_network_directions"] = angles(net)
     return out
 
 
@@ -328,6 +336,9 @@ def bench_c5(args):
     M = sur_render.MinoRenderer(sc)
     mino_ms, mino_ev_ms = time_steps(M.run, args.steps, args.warmup)
     _, mino_solve_ev_ms = time_steps(M.solve, args.steps, 1)
+    M0 = sur_render.MinoRenderer(sc, exit_directions=False)  # crossings and classes only
+    _, mino_solve_noexit_ms = time_steps(M0.solve, args.steps, 1)
+    del M0
     M.run()
     torch.cuda.synchronize()
     fbm = M.fallback_count()
@@ -421,9 +432,11 @@ def bench_c5(args):
             "fallback_rays": fb0, "gate_pass": passes(acc0), "accuracy_vs_plain_fp64": summ(acc0),
         },
         "mino_fallback": {
-            "note": "network-free: Mino-time crossing solver over every pixel + integrator fallback",
+            "note": "network-free: Mino-time crossing solver over every pixel (with the exact escape direction of "
+                    "escaped rays) + integrator fallback",
             "rays_per_s": mino_rate, "ms_per_step": mino_ms, "event_ms_per_step": mino_ev_ms,
             "solve_kernel_ms": mino_solve_ev_ms, "speedup_vs_plain": mino_rate / plain_rate,
+            "solve_kernel_ms_without_escape_directions": mino_solve_noexit_ms,
             "fallback_rays": fbm, "newton_iters_per_ray": it_mino, "gate_pass": passes(accm),
             "accuracy_vs_plain_fp64": summ(accm),
             "escape_direction_vs_plain_fp64": esc_dir_m,
@@ -436,9 +449,11 @@ def bench_c5(args):
             "ratio_network_free_ms_over_surrogate_ms": mino_ms / sur_ms,
             "newton_iters_from_network": it_sur, "newton_iters_from_hermite": it_mino,
             "refine_from_network_ms": refine_render_ev_ms - net_ev_ms,
-            "escape_directions": "the surrogate pipeline's escaped rays carry the network's escape direction "
-                                 "(flagged approximate); the network-free solve leaves it unknown (NaN)",
-            "solve_from_hermite_ms": mino_solve_ev_ms,
+            "escape_directions": "the surrogate pipeline's escaped rays carry the network's direction "
+                                 "(flagged); the network-free pipeline computes the exact one at r_escape "
+                                 "(included in network_free_pipeline_ms, not in solve_from_hermite_ms)",
+            "solve_from_hermite_ms": mino_solve_noexit_ms,
+            "solve_from_hermite_with_escape_directions_ms": mino_solve_ev_ms,
         },
         "ceiling_at_full_bf16_peak": {
             "network_ms_at_2500_tflops": f_sur / (BF16_DENSE_PEAK_TFLOPS * 1e12) * 1000.0,

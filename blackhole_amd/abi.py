@@ -69,7 +69,7 @@ class BH_TraceOptions(C.Structure):
         ("island_r_factor", C.c_double),
         ("island_turn_dist", C.c_double),
         ("blocks_per_cu", C.c_int32),
-        ("pad_launch", C.c_int32),
+        ("mino_exit", C.c_int32),
     ]
 
 

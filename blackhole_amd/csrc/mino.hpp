@@ -102,12 +102,15 @@ MINO_HD double carlson_rf(double x, double y, double z) {
 // observer in the equatorial plane).
 struct PolarTimes {
   double g0, dg;
-  int ok;
+  // the polar motion itself: u(l) = -sgn(u_o) sqrt(wp) sn(k (l - g0) | mpar),
+  // K = K(mpar), s = |u_o| / sqrt(wp)
+  double wp, k, mpar, K, s;
+  int ok, toward;
 };
 
 // u_dir: sign of du/dl at the observer (the tracer's initial v_u = sin(theta_o) beta).
 MINO_HD PolarTimes polar_times(double a, double xi, double eta, double u_o, double u_dir) {
-  PolarTimes t{0.0, 0.0, 0};
+  PolarTimes t{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0};
   if (!(eta > 0.0) || u_o == 0.0) return t;
   const double a2 = a * a;
   const double b = eta + xi * xi - a2;
@@ -123,8 +126,136 @@ MINO_HD PolarTimes polar_times(double a, double xi, double eta, double u_o, doub
   const bool toward = (u_o > 0.0) ? (u_dir < 0.0) : (u_dir > 0.0);
   t.g0 = (toward ? Fo : 2.0 * K - Fo) / k;
   t.dg = 2.0 * K / k;
+  t.wp = wp;
+  t.k = k;
+  t.mpar = mpar;
+  t.K = K;
+  t.s = s;
+  t.toward = toward;
   t.ok = 1;
   return t;
+}
+
+// Carlson's R_C(x, y) = R_F(x, y, y), x >= 0, y > 0: with e = 1 - x / y,
+// R_C = f(e) / sqrt(y), f(e) = asin(sqrt(e)) / sqrt(e) for e > 0 and
+// asinh(sqrt(-e)) / sqrt(-e) for e < 0, and for |e| < 1e-3 their common
+// Maclaurin series 1 + e/6 + 3e^2/40 + 5e^3/112 + 35e^4/1152 + 63e^5/2816
+// (the next term is below 2e-20).
+MINO_HD double carlson_rc(double x, double y) {
+  const double e = 1.0 - x / y;
+  double f;
+  if (fabs(e) < 1e-3) {
+    f = fma(e, fma(e, fma(e, fma(e, fma(e, 63.0 / 2816.0, 35.0 / 1152.0), 5.0 / 112.0), 3.0 / 40.0), 1.0 / 6.0),
+            1.0);
+  } else if (e > 0.0) {
+    const double r = sqrt(e);
+    f = asin(r) / r;
+  } else {
+    const double r = sqrt(-e);
+    f = asinh(r) / r;
+  }
+  return f / sqrt(y);
+}
+
+// Carlson's R_J(x, y, z, p), x, y, z >= 0 (at most one zero), p > 0, by
+// duplication (B. C. Carlson 1995, the algorithm for R_J): each step adds
+// 4^-m R_C(1, 1 + e_m) / d_m with d_m = (sp + sx)(sp + sy)(sp + sz) and
+// e_m = 4^-3m (p - x)(p - y)(p - z) / d_m^2; once the arguments agree to
+// 1e-2 of their mean A, the fifth-order series in X = 1 - x / A, Y, Z and
+// P = -(X + Y + Z) / 2 closes it (truncation of order 1e-12 relative; the
+// square roots are fast_sqrt's).
+MINO_HD double carlson_rj(double x, double y, double z, double p) {
+  const double delta = (p - x) * (p - y) * (p - z);
+  double A = (x + y + z + 2.0 * p) * 0.2;
+  double sum = 0.0, fac = 1.0;
+  for (int it = 0; it < 60; ++it) {
+    const double mx = fmax(fmax(fabs(A - x), fabs(A - y)), fmax(fabs(A - z), fabs(A - p)));
+    if (mx < 1e-2 * A) break;
+    const double sx = fast_sqrt(x), sy = fast_sqrt(y), sz = fast_sqrt(z), sp = fast_sqrt(p);
+    const double lam = sx * (sy + sz) + sy * sz;
+    const double d = (sp + sx) * (sp + sy) * (sp + sz);
+    const double id = 1.0 / d;
+    const double e = fac * fac * fac * delta * (id * id);
+    sum += fac * carlson_rc(1.0, 1.0 + e) * id;
+    x = 0.25 * (x + lam);
+    y = 0.25 * (y + lam);
+    z = 0.25 * (z + lam);
+    p = 0.25 * (p + lam);
+    A = 0.25 * (A + lam);
+    fac *= 0.25;
+  }
+  const double iA = 1.0 / A;
+  const double X = (A - x) * iA, Y = (A - y) * iA, Z = (A - z) * iA;
+  const double P = -0.5 * (X + Y + Z);
+  const double XYZ = X * Y * Z, P2 = P * P;
+  const double E2 = X * Y + X * Z + Y * Z - 3.0 * P2;
+  const double E3 = XYZ + 2.0 * E2 * P + 4.0 * P2 * P;
+  const double E4 = (2.0 * XYZ + E2 * P + 3.0 * P2 * P) * P;
+  const double E5 = XYZ * P2;
+  const double ser = 1.0 - 3.0 / 14.0 * E2 + E3 / 6.0 + 9.0 / 88.0 * E2 * E2 - 3.0 / 22.0 * E4 -
+                     9.0 / 52.0 * E2 * E3 + 3.0 / 26.0 * E5;
+  return fac * ser * iA * inv_sqrt(A) + 6.0 * sum;
+}
+
+// Jacobi sn, cn, dn(v | m) for m < 1.  For 0 <= m < 1 the arithmetic-
+// geometric mean (descending Landen sequence: a_0 = 1, b_0 = sqrt(1 - m),
+// c_0 = sqrt(m), at most eight steps -- c_8 is below 1e-19 for m <= 0.999;
+// a step whose c_n falls below 1e-16 a_n ends the sequence -- then
+// phi_N = 2^N a_N v and phi_{n-1} = (phi_n + asin(c_n sin(phi_n) / a_n)) / 2;
+// sn = sin phi_0, cn = cos phi_0, dn = sqrt(1 - m sn^2)).  For m < 0 the
+// negative-parameter transformation: with mu = -m / (1 - m) and
+// t = sqrt(1 - m), sn(v | m) = sd(t v | mu) / t, cn = cd(t v | mu),
+// dn = nd(t v | mu).  Square roots are fast_sqrt's (1e-13).
+struct Jacobi {
+  double sn, cn, dn;
+};
+
+MINO_HD Jacobi jacobi_snd(double v, double m) {
+  double t = 1.0, mu = m;
+  if (m < 0.0) {
+    t = fast_sqrt(1.0 - m);
+    mu = -m / (1.0 - m);
+    v *= t;
+  }
+  constexpr int kN = 8;
+  double a[kN + 1], c[kN + 1];
+  a[0] = 1.0;
+  c[0] = fast_sqrt(mu);
+  double b = fast_sqrt(1.0 - mu);
+#pragma unroll
+  for (int n = 0; n < kN; ++n) {
+    // once converged, c stays 0 and a fixed: the backward steps only halve
+    const bool done = !(c[n] > 1e-16 * a[n]);
+    a[n + 1] = done ? a[n] : 0.5 * (a[n] + b);
+    c[n + 1] = done ? 0.0 : 0.5 * (a[n] - b);
+    b = done ? b : fast_sqrt(a[n] * b);
+  }
+  double ph = 256.0 * a[kN] * v;
+#pragma unroll
+  for (int n = kN; n >= 1; --n) {
+    if (c[n] != 0.0) ph += asin(c[n] / a[n] * sin(ph));
+    ph *= 0.5;
+  }
+  const double sn = sin(ph), cn = cos(ph), dn = fast_sqrt(1.0 - mu * sn * sn);
+  if (m < 0.0) return Jacobi{sn / (dn * t), cn / dn, 1.0 / dn};
+  return Jacobi{sn, cn, dn};
+}
+
+// Legendre's incomplete integral of the third kind in Jacobi form,
+//   Pi(n; am(v) | m) = integral over [0, v] of dv' / (1 - n sn^2(v' | m)),
+// n < 1, m < 1, for any v: with v = 2 K j + v_r (|v_r| <= K, K = K(m)) it is
+// 2 j Pi(n | m) + Pi(n; am(v_r) | m), the reduced part from sn and cn of v_r
+// (Carlson's form, DLMF 19.25.14 scaled by sin^2):
+//   s R_F(c^2, 1 - m s^2, 1) + n / 3 s^3 R_J(c^2, 1 - m s^2, 1, 1 - n s^2),
+// and the complete one R_F(0, 1 - m, 1) + n / 3 R_J(0, 1 - m, 1, 1 - n).
+MINO_HD double pi3_reduced(double n, double m, double s, double c) {
+  const double s2 = s * s;
+  const double y = 1.0 - m * s2;
+  return s * carlson_rf(c * c, y, 1.0) + n / 3.0 * s * s2 * carlson_rj(c * c, y, 1.0, 1.0 - n * s2);
+}
+
+MINO_HD double pi3_complete(double n, double m) {
+  return carlson_rf(0.0, 1.0 - m, 1.0) + n / 3.0 * carlson_rj(0.0, 1.0 - m, 1.0, 1.0 - n);
 }
 
 // Largest real root of R when R has real roots r3 <= r4 (has34 = 1), from the
@@ -423,6 +554,117 @@ MINO_HD double w_of_r(const Path& p, double r, bool outbound) {
   return x - p.x_o;
 }
 
+// The adaptive Gauss-Kronrod 7-15 march of integrate() over the radial path,
+// carrying beside the Mino time T = integral of f dw the radial part of the
+// azimuth, Phi = integral of a x (2 - a xi x) / (1 - 2x + a^2 x^2) f dw
+// (x = 1 / r: a (2 r - a xi) / Delta per unit Mino time), on the same nodes;
+// a panel is accepted when both estimates pass (Phi with an absolute floor
+// of tol per unit length, for a = 0 or small Phi).
+MINO_HD double integrate_t_phi(const Path& p, double a, double xi, double lo, double hi, double tol, int* ok,
+                               double* phi) {
+  constexpr double xgk[7] = {0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+                             0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+                             0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+                             0.207784955007898467600689403773245};
+  constexpr double wgk[8] = {0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+                             0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+                             0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+                             0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+  constexpr double wg[4] = {0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+                            0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+  const double axi = a * xi, a2 = a * a;
+  double sum = 0.0, sphi = 0.0, x = lo, h = hi - lo;
+  const double hmin = 1e-13 * (hi - lo);
+  for (int it = 0; it < kMaxPanels; ++it) {
+    if (!(x < hi)) {
+      *ok = 1;
+      *phi = sphi;
+      return sum;
+    }
+    if (h > hi - x) h = hi - x;
+    const double c = x + 0.5 * h, hh = 0.5 * h;
+    double tk = 0.0, tg = 0.0, pk = 0.0, pg = 0.0;
+#pragma unroll
+    for (int j = 0; j < 15; ++j) {
+      // node j: offsets -xgk[0..6], 0, +xgk[6..0]
+      const int jj = j < 7 ? j : (j == 7 ? 7 : 14 - j);
+      const double off = j < 7 ? -xgk[j] : (j == 7 ? 0.0 : xgk[14 - j]);
+      const double w = c + hh * off;
+      const double f = path_f(p, w);
+      const double xr = path_x(p, w);
+      const double gf = a * xr * fma(-axi, xr, 2.0) / fma(a2 * xr, xr, fma(-2.0, xr, 1.0)) * f;
+      tk = fma(wgk[jj], f, tk);
+      pk = fma(wgk[jj], gf, pk);
+      if (jj == 7) {
+        tg = fma(wg[3], f, tg);
+        pg = fma(wg[3], gf, pg);
+      } else if (jj & 1) {
+        tg = fma(wg[jj >> 1], f, tg);
+        pg = fma(wg[jj >> 1], gf, pg);
+      }
+    }
+    const double vt = tk * hh, vp = pk * hh;
+    const bool good = fabs((tk - tg) * hh) <= tol * fabs(vt) && fabs((pk - pg) * hh) <= tol * (fabs(vp) + h);
+    if (good || h <= hmin) {
+      sum += vt;
+      sphi += vp;
+      x += h;
+      h *= 2.0;
+    } else {
+      h *= 0.5;
+    }
+  }
+  *ok = !(x < hi);
+  *phi = sphi;
+  return sum;
+}
+
+// The direction (u, phi) where an escaping ray's path ends (r_escape), from
+// the integrals of the tracer's equations in Mino time l, dphi/dl = xi / (1 -
+// u^2) + a (2 r - a xi) / Delta (phi = 0 at the observer):
+//   T_end, phi_r: the radial path's Mino time and radial azimuth
+//       (integrate_t_phi over the inbound and outbound legs);
+//   u(T_end) = -sgn(u_o) sqrt(wp) sn(v_e | mpar), v = k (l - g0), v_e at T_end;
+//   phi_u = xi / k [Pi(wp; am(v_e) | mpar) - Pi(wp; am(v_0) | mpar)],
+//       v_0 = -k g0 (1 - u^2 = 1 - wp sn^2, dl = dv / k);
+// at the observer v_0 = -F_o (toward the equator: sn = -s, cn = sqrt(1 - s^2))
+// or -2K + F_o (away from it: Pi(v_0) = -2 Pi(wp | mpar) + Pi(F_o)).
+// Returns 0 (direction unknown) for a path that does not turn (no escape), a
+// polar motion without equatorial crossings (eta <= 0, an observer in the
+// equatorial plane) or a failed quadrature.  Not inlined on the device: it
+// runs once per escaped ray, and inlined its registers would spill inside
+// the crossing solve's loops.
+#if defined(__HIPCC__)
+__host__ __device__ __attribute__((noinline))
+#else
+inline
+#endif
+int exit_direction(const Path& p, const PolarTimes& pt, double a, double xi, double u_o, double tol,
+                   double* u_exit, double* phi_exit) {
+  if (!p.turning || !pt.ok) return 0;
+  int ok = 1;
+  double phi_in = 0.0, phi_out = 0.0;
+  double T_end = integrate_t_phi(p, a, xi, 0.0, p.s_o, tol, &ok, &phi_in);
+  if (!ok) return 0;
+  T_end += integrate_t_phi(p, a, xi, p.s_o, p.w_end, tol, &ok, &phi_out);
+  if (!ok) return 0;
+  const double twoK = 2.0 * pt.K;
+  const double ve = pt.k * (T_end - pt.g0);
+  const double je = rint(ve / twoK);
+  const Jacobi J = jacobi_snd(fma(-twoK, je, ve), pt.mpar);
+  const long long ji = static_cast<long long>(je);
+  const double sn_e = (ji & 1) ? -J.sn : J.sn;  // sn(v + 2K) = -sn(v)
+  const double c0 = fast_sqrt(fmax(1.0 - pt.s * pt.s, 0.0));
+  const double Pc = pi3_complete(pt.wp, pt.mpar);
+  const double Po = pi3_reduced(pt.wp, pt.mpar, pt.s, c0);
+  const double P0 = pt.toward ? -Po : fma(-2.0, Pc, Po);
+  const double Pe = fma(2.0 * je, Pc, pi3_reduced(pt.wp, pt.mpar, J.sn, J.cn));
+  const double sq_wp = fast_sqrt(pt.wp);
+  *u_exit = (u_o > 0.0 ? -sq_wp : sq_wp) * sn_e;
+  *phi_exit = (phi_in + phi_out) + xi / pt.k * (Pe - P0);
+  return 1;
+}
+
 // Disk classes as the tracer's records number them.
 constexpr int kHorizon = 1, kDisk = 2, kEscaped = 3;
 constexpr int kMaxCrossings = 16;
@@ -432,7 +674,8 @@ constexpr int kMaxCrossings = 16;
 // disk), the stopping radius r_stop or the escape radius r_escape.
 // guess[0 .. n_guess) are starting radii for the first crossings (e.g. a
 // learned estimate); without them each crossing starts from the linear
-// interpolant of its bracket.
+// interpolant of its bracket.  EXIT: an escaped ray also gets its end
+// direction (exit_direction).
 struct RaySolution {
   double r[3];   // first three crossing radii (0 beyond n)
   double r_hit;  // radius of the disk crossing (0 without one)
@@ -441,12 +684,16 @@ struct RaySolution {
   int cls;       // kHorizon, kDisk or kEscaped
   int iters;     // Newton iterations over all crossings
   int ok;        // 0: the quadrature or the iteration failed
+  // EXIT: the exact end direction of an escaped ray at r_escape (exit_ok = 1)
+  double u_exit, phi_exit;
+  int exit_ok;
 };
 
+template <bool EXIT = false>
 MINO_HD RaySolution solve_ray(double a, double xi, double eta, double r_o, double u_o, double u_dir,
                               double r_stop, double r_escape, double r_in, double r_out, const double* guess,
                               int n_guess, double tol) {
-  RaySolution out{{0.0, 0.0, 0.0}, 0.0, 0, -1, kHorizon, 0, 0};
+  RaySolution out{{0.0, 0.0, 0.0}, 0.0, 0, -1, kHorizon, 0, 0, 0.0, 0.0, 0};
   Path p;
   if (!make_path(a, xi, eta, r_o, r_stop, r_escape, &p)) return out;
   const PolarTimes pt = polar_times(a, xi, eta, u_o, u_dir);
@@ -492,6 +739,7 @@ MINO_HD RaySolution solve_ray(double a, double xi, double eta, double r_o, doubl
       }
       if (!(G < T_total)) {  // the path ends (escape or capture) before this crossing
         out.ok = 1;
+        if (EXIT && p.turning) out.exit_ok = exit_direction(p, pt, a, xi, u_o, tol, &out.u_exit, &out.phi_exit);
         return out;
       }
       hi = p.w_end;

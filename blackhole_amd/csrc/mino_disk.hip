@@ -11,10 +11,13 @@
 // crossing radii, g from the exact Keplerian formula with the ray's xi (0
 // inside r_ISCO), the intensity of the crossing inside [r_ISCO, r_out] from
 // the camera's LUT by the integrator's formula, the hit class, the number of
-// crossings, steps = Newton iterations, flags BH_FLAG_REFINED.  The solve
-// does not produce the path's end direction: u_exit and phi_exit are NaN,
-// except in surrogate records the solve finds escaping, which keep the
-// network's escape direction and are flagged BH_FLAG_EXIT_NETWORK.
+// crossings, steps = Newton iterations, flags BH_FLAG_REFINED.  Escaped
+// rays end at r_exit = r_escape; when asked for (the EXIT instantiation) with
+// their exact direction there (u_exit, phi_exit from the polar motion's
+// Jacobi functions and the azimuth integrals: mino::exit_direction).
+// Captured and disk rays have no end direction (NaN); an escaping surrogate
+// record without the exact one keeps the network's (flagged
+// BH_FLAG_EXIT_NETWORK).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,6 +35,9 @@
 namespace bhk {
 int fail(const std::string& msg);
 
+// EXIT: escaped rays also get their exact end direction (an instantiation of
+// its own, so that the crossing solve without it keeps its registers).
+template <bool EXIT>
 __global__ void __launch_bounds__(256, 4) k_mino_disk(const MinoParams P) {
   const long long q = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   bool fb = false;
@@ -71,8 +77,8 @@ __global__ void __launch_bounds__(256, 4) k_mino_disk(const MinoParams P) {
         guess[2] = cr.z;
         n_guess = min((tail.x >> 8) & 0xff, 3);
       }
-      const mino::RaySolution s = mino::solve_ray(a, xi, eta, k.r_obs, k.cos_o, k.sin_o * beta, k.r_stop,
-                                                  P.r_escape, k.r_isco, P.disk_r_out, guess, n_guess, P.tol);
+      const mino::RaySolution s = mino::solve_ray<EXIT>(a, xi, eta, k.r_obs, k.cos_o, k.sin_o * beta, k.r_stop,
+                                                        P.r_escape, k.r_isco, P.disk_r_out, guess, n_guess, P.tol);
       if (s.ok) {
         double g[3];
 #pragma unroll
@@ -87,13 +93,15 @@ __global__ void __launch_bounds__(256, 4) k_mino_disk(const MinoParams P) {
         } else {
           r_exit = (s.cls == mino::kHorizon) ? k.r_stop : P.r_escape;
         }
-        // the solve finds crossings and the class, not the path's end point:
-        // an escaping surrogate ray keeps the network's escape direction
-        // (flagged BH_FLAG_EXIT_NETWORK); otherwise u_exit and phi_exit are
-        // marked unknown (NaN)
-        const bool exit_net = P.refine && (flags_in & BH_FLAG_SURROGATE) && s.cls == mino::kEscaped;
-        const double2 dir =
-            exit_net ? d2[1] : make_double2(__builtin_nan(""), __builtin_nan(""));
+        // the path's end direction: exact for an escaped ray with EXIT; else an
+        // escaping surrogate ray keeps the network's (flagged
+        // BH_FLAG_EXIT_NETWORK), and the others are marked unknown (NaN)
+        const bool esc = s.cls == mino::kEscaped;
+        const bool exact = EXIT && esc && s.exit_ok;
+        const bool exit_net = !exact && esc && P.refine && (flags_in & BH_FLAG_SURROGATE);
+        const double2 dir = exact      ? make_double2(s.u_exit, s.phi_exit)
+                            : exit_net ? d2[1]
+                                       : make_double2(__builtin_nan(""), __builtin_nan(""));
         d2[0] = make_double2(I, r_exit);
         d2[1] = dir;
         d2[2] = make_double2(s.r[0], s.r[1]);
@@ -131,7 +139,10 @@ __global__ void k_mino_zero(unsigned long long* count) {
 hipError_t launch_mino_disk(const MinoParams& P, hipStream_t s) {
   const long long nblk = (P.n_total + 255) / 256;
   if (nblk <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mino_disk, dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, P);
+  if (P.exit_dir)
+    hipLaunchKernelGGL(k_mino_disk<true>, dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, P);
+  else
+    hipLaunchKernelGGL(k_mino_disk<false>, dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s, P);
   return hipGetLastError();
 }
 
@@ -164,6 +175,7 @@ int mino_entry(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts, c
   if (!opts->disk_enabled || opts->volume_enabled)
     return fail(w + ": thin disk only (disk_enabled = 1, volume_enabled = 0)");
   if (!(opts->disk_r_out > 0.0) || !(opts->r_escape > 0.0)) return fail(w + ": disk_r_out and r_escape must be > 0");
+  if (opts->mino_exit < 0 || opts->mino_exit > 2) return fail(w + ": mino_exit must be 0, 1 or 2");
   std::vector<CamDev> hc;
   long long total = 0;
   if (make_cam_devs(cams, n_cams, opts->horizon_factor, who, &hc, &total) != 0) return -1;
@@ -205,6 +217,7 @@ int mino_entry(const BH_Camera* cams, int n_cams, const BH_TraceOptions* opts, c
   P.n_cams = n_cams;
   P.lut_n = disk_lut_n;
   P.refine = refine;
+  P.exit_dir = opts->mino_exit == 1 || (opts->mino_exit == 0 && !refine);
   if (total / 256 + 1 > 0x7fffffffll) return fail(w + ": too many pixels");
   if (!refine) hipLaunchKernelGGL(k_mino_zero, dim3(1), dim3(64), 0, st, P.fb_count);
   const hipError_t e = launch_mino_disk(P, st);

@@ -656,6 +656,7 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
   if (!(rp->ring_band >= 0.0f && rp->edge_tol >= 0.0f && rp->ambiguity >= 0.0f) || rp->audit_every < 0 ||
       rp->refine < 0 || rp->refine > 3)
     return fail("bh_surrogate_render: invalid render parameters");
+  if (opts->mino_exit < 0 || opts->mino_exit > 2) return fail("bh_surrogate_render: mino_exit must be 0, 1 or 2");
   const SurView v = surrogate_view(s);
   if (v.width != kW || v.L < 2 || v.L > kMaxLayers)
     return fail("bh_surrogate_render: the fused kernel needs width 256 and 0..6 hidden 256 x 256 layers");
@@ -749,6 +750,7 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
     M.n_cams = n_cams;
     M.lut_n = disk_lut_n;
     M.refine = 1;
+    M.exit_dir = opts->mino_exit == 1;
     e = launch_mino_disk(M, st);
   }
   return e == hipSuccess ? 0 : fail(std::string("bh_surrogate_render: launch failed: ") + hipGetErrorString(e));

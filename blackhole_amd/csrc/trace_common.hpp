@@ -238,7 +238,7 @@ struct MinoParams {
   unsigned long long* fb_count;
   long long fb_cap, n_total;
   double disk_r_out, nu, T_max, r_escape, tol;
-  int n_cams, lut_n, refine, pad;
+  int n_cams, lut_n, refine, exit_dir;  // exit_dir: exact escape directions of escaped rays
 };
 // Relative tolerance on the |K15 - G7| estimate of each radial quadrature panel.  The
 // estimate is far above the Kronrod rule's actual error for these smooth integrands: at

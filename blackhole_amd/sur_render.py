@@ -50,6 +50,9 @@ class RenderConfig:
     # against 57.0 - 57.1 ms in sequence; beside the exact solve instead it was slower)
     overlap_audit: bool = True
     intensity_floor_rel: float = 1e-6  # audit compares pixels with I_ref > floor x max I
+    # escape directions of escaped rays: the network's (flagged BH_FLAG_EXIT_NETWORK) by
+    # default; True: exact from the refinement's Mino-time solve (BH_TraceOptions.mino_exit = 1)
+    exact_exit: bool = False
 
 
 def ring_tables(cams) -> np.ndarray:
@@ -68,6 +71,9 @@ class SurrogateRenderer:
         import torch
         self.torch = torch
         self.sur, self.sc, self.cfg = sur, scene, cfg
+        # the scene's options with this renderer's escape-direction mode
+        self.opts = abi.BH_TraceOptions.from_buffer_copy(scene.opts)
+        self.opts.mino_exit = 1 if cfg.exact_exit else 0
         n = scene.n_records
         self.n = n
         dev = scene.luts.device
@@ -103,7 +109,7 @@ class SurrogateRenderer:
         cam_arr = (abi.BH_Camera * len(self.sc.cams))(*self.sc.cams)
         p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
         check(lib().bh_trace_disk_mino_refine(
-            cam_arr, len(self.sc.cams), C.byref(self.sc.opts), p(self.sc.luts), self.sc.lut_n, p(self.out),
+            cam_arr, len(self.sc.cams), C.byref(self.opts), p(self.sc.luts), self.sc.lut_n, p(self.out),
             p(self.fb_ids), p(self.counts[0:1]), self.fb_cap, self._stream()), "bh_trace_disk_mino_refine")
 
     def _network(self) -> None:
@@ -111,7 +117,7 @@ class SurrogateRenderer:
         L = S._sigs(lib())
         p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
         check(L.bh_surrogate_render(
-            self.sur.h, cam_arr, len(self.sc.cams), C.byref(self.sc.opts), p(self.sc.luts), self.sc.lut_n,
+            self.sur.h, cam_arr, len(self.sc.cams), C.byref(self.opts), p(self.sc.luts), self.sc.lut_n,
             p(self.ring), C.byref(self.params), p(self.out), p(self.fb_ids), p(self.counts[0:1]),
             self.fb_cap, p(self.audit_ids), p(self.counts[1:2]), self.audit_cap, self._stream()),
             "bh_surrogate_render")
@@ -174,13 +180,17 @@ class SurrogateRenderer:
 
 class MinoRenderer:
     """Network-free thin-disk images: the Mino-time crossing solver over every
-    pixel (bh_trace_disk_mino), then the plain integrator over the rays the
-    solver leaves (bh_trace_pixel_list into the same records)."""
+    pixel (bh_trace_disk_mino, with the exact escape direction of escaped rays
+    unless exit_directions is False), then the plain integrator over the rays
+    the solver leaves (bh_trace_pixel_list into the same records)."""
 
-    def __init__(self, scene: tracer.DeviceScene):
+    def __init__(self, scene: tracer.DeviceScene, exit_directions: bool = True):
         import torch
         self.torch = torch
         self.sc = scene
+        # exact escape directions of escaped rays (BH_TraceOptions.mino_exit 0), or none (2)
+        self.opts = abi.BH_TraceOptions.from_buffer_copy(scene.opts)
+        self.opts.mino_exit = 0 if exit_directions else 2
         n = scene.n_records
         dev = scene.luts.device
         self.out = tracer.alloc_records(n, device=dev)
@@ -188,7 +198,7 @@ class MinoRenderer:
         self.count = torch.zeros(1, dtype=torch.int64, device=dev)
 
     def solve(self) -> None:
-        tracer.trace_disk_mino(self.sc, self.out, self.fb_ids, self.count)
+        tracer.trace_disk_mino(self.sc, self.out, self.fb_ids, self.count, opts=self.opts)
 
     def fallback(self) -> None:
         tracer.launch_ids(self.sc, self.fb_ids, self.count, self.out)

@@ -192,17 +192,18 @@ def launch_ids(sc: DeviceScene, ids, count, out, stream=None, compact: bool = Fa
     check(rc, "bh_trace_pixel_list_compact" if compact else "bh_trace_pixel_list")
 
 
-def trace_disk_mino(sc: DeviceScene, out, fb_ids, fb_count, stream=None) -> None:
+def trace_disk_mino(sc: DeviceScene, out, fb_ids, fb_count, stream=None, opts=None) -> None:
     """Asynchronous thin-disk records of every pixel from the Mino-time
     crossing solver (bh_trace_disk_mino) into out; ids of unresolved rays to
     fb_ids (int64 cuda) with their number in fb_count (int64 cuda, one
-    element, zeroed by the call)."""
+    element, zeroed by the call).  opts: the scene's options by default."""
     torch = _torch()
     assert out.dtype == torch.uint8 and out.is_cuda and out.shape[1] == 128
     assert out.shape[0] >= sc.n_records and sc.luts is not None
     cam_arr = (abi.BH_Camera * len(sc.cams))(*sc.cams)
     s = stream if stream is not None else torch.cuda.current_stream()
-    rc = lib().bh_trace_disk_mino(cam_arr, len(sc.cams), C.byref(sc.opts), C.c_void_p(sc.luts.data_ptr()),
+    o = sc.opts if opts is None else opts
+    rc = lib().bh_trace_disk_mino(cam_arr, len(sc.cams), C.byref(o), C.c_void_p(sc.luts.data_ptr()),
                                   sc.lut_n, C.c_void_p(out.data_ptr()), C.c_void_p(fb_ids.data_ptr()),
                                   C.c_void_p(fb_count.data_ptr()), int(fb_ids.numel()), C.c_void_p(s.cuda_stream))
     check(rc, "bh_trace_disk_mino")

@@ -136,7 +136,13 @@ typedef struct BH_TraceOptions {
     int32_t blocks_per_cu;  /* persistent-kernel workgroups per CU (0 = the
                                autotuned value for this device and kernel if
                                one is cached, else the occupancy limit)         */
-    int32_t pad_launch;
+    int32_t mino_exit;      /* Mino-time solver (bh_trace_disk_mino[_refine], the
+                               surrogate render's refinement): escape directions
+                               of escaped rays.  0 (default): exact in
+                               bh_trace_disk_mino, the surrogate network's
+                               (flagged BH_FLAG_EXIT_NETWORK) in a refinement;
+                               1: exact in both; 2: none computed (NaN, or the
+                               network's in a refinement)                       */
 } BH_TraceOptions;
 
 /* Synthetic GRMHD-like volume on a Boyer-Lindquist (r, theta, phi) grid.
@@ -386,11 +392,15 @@ int bh_trace_image_plane_pol(const BH_Camera* cams, int n_cams, const BH_TraceOp
  * bracketed Newton iteration), then the integrator's disk record (hit class,
  * first three crossing radii, Keplerian g, LUT intensity; steps = Newton
  * iterations, flags BH_FLAG_REFINED).  Same cameras, options and LUTs as
- * bh_trace_image_plane with disk_enabled = 1 and volume_enabled = 0; the
- * path's end point direction (u_exit, phi_exit) is not computed and is
- * written as NaN (in records this solver refines, an escaping ray keeps the
- * surrogate network's escape direction instead, flagged
- * BH_FLAG_EXIT_NETWORK).  Rays the solver
+ * bh_trace_image_plane with disk_enabled = 1 and volume_enabled = 0.  An
+ * escaped ray ends at r_exit = r_escape with its exact direction there
+ * (u_exit, phi_exit: Jacobi sn of the polar motion and the azimuth
+ * integrals, Legendre's Pi by Carlson's R_F / R_J and the radial part by
+ * quadrature) unless opts->mino_exit = 2; captured and disk rays have no
+ * end direction (NaN).  In records this solver refines (mino_exit 0), an
+ * escaping ray keeps the surrogate network's direction instead, flagged
+ * BH_FLAG_EXIT_NETWORK; mino_exit = 1 computes the exact one there too.
+ * Rays the solver
  * does not resolve (within about 1e-3 M of the critical curve) get
  * BH_FLAG_FALLBACK and their ids in fallback_ids (count in device memory,
  * zeroed by the call) for bh_trace_pixel_list.  No counterpart in the

@@ -125,6 +125,11 @@ def test_mino_entries_validation_without_gpu():
         ov = tracer.default_options(disk_enabled=1, volume_enabled=1, disk_r_out=20.0)
         assert fn(cams, 1, C.byref(ov), p1, 256, p1, p1, p1, 16, None) == -1       # volumes not supported
         assert b"thin disk only" in L.bh_last_error()
+        ox = tracer.default_options(disk_enabled=1, volume_enabled=0, disk_r_out=20.0)
+        assert ox.mino_exit == 0
+        ox.mino_exit = 3
+        assert fn(cams, 1, C.byref(ox), p1, 256, p1, p1, p1, 16, None) == -1       # unknown escape-direction mode
+        assert b"mino_exit" in L.bh_last_error()
 
 
 def test_measure_peaks_validation_without_gpu():

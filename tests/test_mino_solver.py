@@ -10,7 +10,13 @@ compiled by g++ from the same header the GPU kernel includes:
     degree 5 and not at 6 (a wrong node or weight breaks one of these);
   * whole rays against the oracle's step-integrated records on the 16 frames
     of the surrogate comparison (spins 0/0.5/0.9/0.998 x 17/40/60/85 deg) at
-    low resolution: identical hit classes, crossing radii within 1e-6.
+    low resolution: identical hit classes, crossing radii within 1e-6;
+  * the escape direction's pieces -- Carlson's R_C and R_J against scipy's
+    elliprc / elliprj, Jacobi sn, cn, dn against scipy's ellipj (m >= 0) and,
+    for the polar motion's negative parameters, by F(am(v) | m) = v with
+    scipy's ellipkinc and the identities sn^2 + cn^2 = 1, dn^2 = 1 - m sn^2,
+    Legendre's Pi against direct quadrature -- and the direction itself
+    against the oracle's records at the oracle's own exit radius.
 """
 from __future__ import annotations
 
@@ -43,6 +49,12 @@ def mino(tmp_path_factory):
     L.mino_radial_roots.argtypes = [_D] * 3 + [C.POINTER(_D)]
     L.mino_polar_times.argtypes = [_D] * 5 + [C.POINTER(_D)]
     L.mino_solve_pixel.argtypes = [_D] * 9 + [C.POINTER(_D), C.c_int, _D, C.POINTER(_D), C.POINTER(C.c_int)]
+    for name, n in (("mino_carlson_rc", 2), ("mino_carlson_rj", 4), ("mino_pi3_reduced", 4), ("mino_pi3_complete", 2)):
+        getattr(L, name).restype = _D
+        getattr(L, name).argtypes = [_D] * n
+    L.mino_jacobi.argtypes = [_D, _D, C.POINTER(_D)]
+    L.mino_solve_pixel_exit.restype = C.c_int
+    L.mino_solve_pixel_exit.argtypes = [_D] * 10 + [C.POINTER(_D), C.POINTER(C.c_int), C.POINTER(_D)]
     return L
 
 
@@ -147,3 +159,84 @@ def test_rays_match_the_step_integrator(mino):
                     worst = max(worst, abs(r[k] - rec["cross_r"][k]) / rec["cross_r"][k])
     assert fails <= n // 1000
     assert worst <= 1e-6
+
+
+def test_carlson_rc_rj_match_scipy(mino):
+    rng = np.random.default_rng(21)
+    worst_c = worst_j = 0.0
+    for _ in range(3000):
+        x, y = rng.uniform(0.0, 10.0), rng.uniform(0.01, 10.0)
+        if rng.random() < 0.3:
+            y = x * (1.0 + rng.uniform(-2e-3, 2e-3)) + 1e-9  # the series branch near x = y
+        worst_c = max(worst_c, abs(mino.mino_carlson_rc(x, y) / special.elliprc(x, y) - 1.0))
+        x, y, z = rng.uniform(0.0, 10.0, 3)
+        if rng.random() < 0.2:
+            x = 0.0
+        p = rng.uniform(0.01, 10.0)
+        worst_j = max(worst_j, abs(mino.mino_carlson_rj(x, y, z, p) / special.elliprj(x, y, z, p) - 1.0))
+    assert worst_c <= 1e-14 and worst_j <= 1e-14, (worst_c, worst_j)
+
+
+def test_jacobi_functions(mino):
+    rng = np.random.default_rng(22)
+    out = (_D * 3)()
+    for _ in range(2000):
+        m, v = rng.uniform(0.0, 0.999), rng.uniform(-4.0, 4.0)
+        mino.mino_jacobi(v, m, out)
+        sn, cn, dn, _ = special.ellipj(v, m)
+        assert max(abs(out[0] - sn), abs(out[1] - cn), abs(out[2] - dn)) <= 1e-14, (m, v)
+    for _ in range(2000):
+        m = -rng.uniform(0.0, 60.0)
+        K = special.ellipk(m)
+        v = rng.uniform(-K, K)
+        mino.mino_jacobi(v, m, out)
+        am = math.atan2(out[0], out[1])
+        assert abs(special.ellipkinc(am, m) - v) <= 1e-13 * max(1.0, K), (m, v)
+        assert abs(out[0] ** 2 + out[1] ** 2 - 1.0) <= 5e-14 and abs(out[2] ** 2 - (1.0 - m * out[0] ** 2)) <= 1e-13
+
+
+def test_legendre_pi_matches_quadrature(mino):
+    rng = np.random.default_rng(23)
+    for _ in range(300):
+        n = rng.uniform(0.0, 0.999)
+        m = -rng.uniform(0.0, 40.0) if rng.random() < 0.8 else rng.uniform(0.0, 0.9)
+        ph = rng.uniform(-math.pi / 2, math.pi / 2)
+        f = lambda t: 1.0 / ((1.0 - n * math.sin(t) ** 2) * math.sqrt(1.0 - m * math.sin(t) ** 2))  # noqa: E731
+        want = integrate.quad(f, 0.0, ph, epsabs=0, epsrel=1e-13, limit=200)[0]
+        got = mino.mino_pi3_reduced(n, m, math.sin(ph), math.cos(ph))
+        assert abs(got - want) <= 1e-13 * abs(want) + 1e-300, (n, m, ph)
+        wc = integrate.quad(f, 0.0, math.pi / 2, epsabs=0, epsrel=1e-13, limit=200)[0]
+        assert abs(mino.mino_pi3_complete(n, m) - wc) <= 1e-13 * wc
+
+
+def test_escape_direction_matches_the_step_integrator(mino):
+    """u_exit and phi_exit of escaped rays at the oracle's own exit radius (the
+    integrator stops at the end of the step that passes r_escape): within the
+    integrator's accumulated error."""
+    w = workloads.c5(24)
+    o = w["opts"]
+    ref = OL.trace(w["cams"], o, luts=w["luts"], lut_n=w["luts"].shape[1], nthreads=8)
+    r, ints, d = (_D * 3)(), (C.c_int * 6)(), (_D * 2)()
+    du, dphi, n, fails = [], [], 0, 0
+    for cam in w["cams"]:
+        r_stop = o.horizon_factor * tracer.outer_horizon(cam.spin)
+        r_in = tracer.isco(cam.spin)
+        for j in range(cam.height):
+            for i in range(cam.width):
+                rec = ref[cam.record_offset + j * cam.width + i]
+                if rec["hit_class"] != 3:  # escaped
+                    continue
+                al = cam.alpha_min + (i + 0.5) * (cam.alpha_max - cam.alpha_min) / cam.width
+                be = cam.beta_max - (j + 0.5) * (cam.beta_max - cam.beta_min) / cam.height
+                ok = mino.mino_solve_pixel_exit(cam.spin, cam.theta_obs, cam.r_obs, al, be, r_stop, rec["r_exit"],
+                                                r_in, o.disk_r_out, 1e-10, r, ints, d)
+                n += 1
+                if not ok or ints[2] != 3:
+                    fails += 1
+                    continue
+                du.append(abs(d[0] - rec["u_exit"]))
+                dphi.append(abs(d[1] - rec["phi_exit"]))
+    du, dphi = np.array(du), np.array(dphi)
+    print("escape direction vs oracle:", n, "rays; |du| max", du.max(), "|dphi| max", dphi.max(), "mean", dphi.mean())
+    assert n > 3000 and fails <= n // 1000
+    assert du.max() <= 1e-6 and dphi.max() <= 2e-5 and dphi.mean() <= 1e-6
