@@ -646,16 +646,9 @@ MINO_HD double integrate_t_phi(const Path& p, double a, double xi, double lo, do
 // or -2K + F_o (away from it: Pi(v_0) = -2 Pi(wp | mpar) + Pi(F_o)).
 // Returns 0 (direction unknown) for a path that does not turn (no escape), a
 // polar motion without equatorial crossings (eta <= 0, an observer in the
-// equatorial plane) or a failed quadrature.  Not inlined on the device: it
-// runs once per escaped ray, and inlined its registers would spill inside
-// the crossing solve's loops.
-#if defined(__HIPCC__)
-__host__ __device__ __attribute__((noinline))
-#else
-inline
-#endif
-int exit_direction(const Path& p, const PolarTimes& pt, double a, double xi, double u_o, double tol,
-                   double* u_exit, double* phi_exit) {
+// equatorial plane) or a failed quadrature.
+MINO_HD int exit_direction(const Path& p, const PolarTimes& pt, double a, double xi, double u_o, double tol,
+                           double* u_exit, double* phi_exit) {
   if (!p.turning || !pt.ok) return 0;
   int ok = 1;
   double phi_in = 0.0, phi_out = 0.0;
