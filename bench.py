@@ -327,6 +327,7 @@ def bench_c5(args):
     acc = frame_accuracy(R.out, plain, w["cams"])
     esc_dir = exit_direction_accuracy(R.out, plain)
     refined = (R.out[:, 114] & 0x10) != 0
+    n_kept = int(((R.out[:, 114] & 0x40) != 0).sum())  # network records kept unsolved
     iters = R.out.view(torch.int32)[:, 26]
     it_sur = float(iters[refined].double().mean())
     del R
@@ -422,6 +423,7 @@ def bench_c5(args):
             "render_kernel_ms": net_ev_ms, "render_plus_refine_ms": refine_render_ev_ms,
             "network_alone_render_ms": render_ev_ms,
             "fallback_rays": fb, "fallback_frac": fb / n, "audit_rays": au, "newton_iters_per_ray": it_sur,
+            "network_kept_rays": n_kept, "network_kept_frac": n_kept / n,
             "gate_pass": gate, "accuracy_vs_plain_fp64": summ(acc),
             "escape_direction_vs_plain_fp64": esc_dir,
             "audit": {k: audit[k] for k in ("n", "mean_rel", "max_rel", "class_diff", "flux_weighted")},

@@ -21,6 +21,7 @@ FLAG_AUDITED = 0x04
 FLAG_SURROGATE = 0x08
 FLAG_REFINED = 0x10
 FLAG_EXIT_NETWORK = 0x20
+FLAG_NETWORK_KEPT = 0x40
 
 
 class BH_Camera(C.Structure):

@@ -63,7 +63,9 @@ __global__ void __launch_bounds__(256, 4) k_mino_disk(const MinoParams P) {
     int4 tail = make_int4(0, __float_as_int(1.0f), 0, 0);  // class/count/flags, confidence, fp64 steps, samples
     if (P.refine) tail = *reinterpret_cast<const int4*>(i4 + 28);
     const unsigned flags_in = (static_cast<unsigned>(tail.x) >> 16) & 0xffu;
-    if (local >= 0 && local < k.w * k.h && !(flags_in & BH_FLAG_FALLBACK)) {
+    // fallback rays go to the integrator; rays whose network record is kept
+    // (classified with margin away from the disk) are not solved
+    if (local >= 0 && local < k.w * k.h && !(flags_in & (BH_FLAG_FALLBACK | BH_FLAG_NETWORK_KEPT))) {
       const long long py = local / k.w, px = local - py * k.w;
       const double alpha = k.alpha_min + (static_cast<double>(px) + 0.5) * k.dalpha;
       const double beta = k.beta_max - (static_cast<double>(py) + 0.5) * k.dbeta;

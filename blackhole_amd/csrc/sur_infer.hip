@@ -102,6 +102,8 @@ struct InferParams {
   float ring_band, edge_tol, conf_max, amb;
   int refine;      // the Mino-time solver follows: only the ring band falls back here
   int band_solve;  // ... and solves the ring band too (no crossings written, nothing listed)
+  double keep_r;   // with the solve to follow: rays escaped or captured with every predicted
+                   // crossing at r >= keep_r keep the network's record (0: none)
 };
 
 __device__ __forceinline__ uint32_t hash32(uint64_t x) {
@@ -369,6 +371,15 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
       band = d < P.ring_band * obl;
       if (band && !P.band_solve) fb = true;
     }
+    // with the solve to follow, a ray the network places escaped or captured
+    // with an unambiguous count and capture and every predicted crossing
+    // beyond keep_r keeps the network's record (its intensity is 0 whatever
+    // the radii); the margin makes the class safe
+    bool keep = P.refine && P.keep_r > 0.0 && !band && fabsf(ncf - rintf(ncf)) <= P.amb &&
+                fabsf(cap - rintf(cap)) <= P.amb;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (c < nc) keep = keep && exp(static_cast<double>(O[2 * c] + lnbase)) >= P.keep_r;
     double cr0 = 0.0, cr1 = 0.0, cr2 = 0.0, cg0 = 0.0, cg1 = 0.0, cg2 = 0.0, I = 0.0;
     int hit = -1;
 #pragma unroll
@@ -376,8 +387,8 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
       if (c < nc && hit < 0) {
         const double r = exp(static_cast<double>(O[2 * c] + lnbase));
         // with the solve to follow, g and the intensity are written by the solve
-        // (or the integrator) for every ray, so only the radii are decoded
-        const double g = (!P.refine && r >= k.r_isco) ? disk_g(r, a, xi) : 0.0;
+        // (or the integrator) for every other ray, so only the radii are decoded
+        const double g = ((!P.refine || keep) && r >= k.r_isco) ? disk_g(r, a, xi) : 0.0;
         if (c == 0) { cr0 = r; cg0 = g; }
         if (c == 1) { cr1 = r; cg1 = g; }
         if (c == 2) { cr2 = r; cg2 = g; }
@@ -422,7 +433,8 @@ __global__ void __launch_bounds__(256, 2) k_sur_render(const InferParams P) {
     d2[5] = make_double2(0.0, 0.0);
     int* i4 = reinterpret_cast<int*>(rec);
     *reinterpret_cast<int4*>(i4 + 24) = make_int4(0, 0, 0, 0);  // drift_carter, steps, attempts
-    const int flags = BH_FLAG_SURROGATE | (fb ? BH_FLAG_FALLBACK : 0);
+    const int flags = BH_FLAG_SURROGATE | (fb ? BH_FLAG_FALLBACK : 0) |
+                      (keep ? BH_FLAG_NETWORK_KEPT | (cls == BH_HIT_ESCAPED ? BH_FLAG_EXIT_NETWORK : 0) : 0);
     *reinterpret_cast<int4*>(i4 + 28) = make_int4(cls | (ncr << 8) | (flags << 16), __float_as_int(conf), 0, 0);
     aud = !fb && P.audit_every > 0 &&
           (hash32(static_cast<uint64_t>(gid)) % static_cast<uint32_t>(P.audit_every)) == 0u;
@@ -654,7 +666,7 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
     return fail("bh_surrogate_render: NULL argument");
   if (fallback_cap <= 0 || audit_cap <= 0) return fail("bh_surrogate_render: list capacities must be > 0");
   if (!(rp->ring_band >= 0.0f && rp->edge_tol >= 0.0f && rp->ambiguity >= 0.0f) || rp->audit_every < 0 ||
-      rp->refine < 0 || rp->refine > 3)
+      rp->refine < 0 || rp->refine > 3 || !(rp->keep_factor == 0.0f || rp->keep_factor >= 1.0f))
     return fail("bh_surrogate_render: invalid render parameters");
   if (opts->mino_exit < 0 || opts->mino_exit > 2) return fail("bh_surrogate_render: mino_exit must be 0, 1 or 2");
   const SurView v = surrogate_view(s);
@@ -727,6 +739,7 @@ int bh_surrogate_render(BH_Surrogate* s, const BH_Camera* cams, int n_cams, cons
   P.amb = rp->ambiguity;
   P.refine = rp->refine != 0;
   P.band_solve = rp->refine == 3;
+  P.keep_r = (P.refine && opts->mino_exit != 1) ? static_cast<double>(rp->keep_factor) * opts->disk_r_out : 0.0;
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, P.fb_count, P.audit_count);
   const long long nblk = (total + kTile - 1) / kTile;
   if (nblk > 0x7fffffffll) return fail("bh_surrogate_render: too many pixels");

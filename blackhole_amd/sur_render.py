@@ -51,8 +51,13 @@ class RenderConfig:
     overlap_audit: bool = True
     intensity_floor_rel: float = 1e-6  # audit compares pixels with I_ref > floor x max I
     # escape directions of escaped rays: the network's (flagged BH_FLAG_EXIT_NETWORK) by
-    # default; True: exact from the refinement's Mino-time solve (BH_TraceOptions.mino_exit = 1)
+    # default; True: exact from the refinement's Mino-time solve (BH_TraceOptions.mino_exit = 1;
+    # every ray is then solved, keep_factor does not apply)
     exact_exit: bool = False
+    # with refine: rays the network places escaped or captured with every predicted crossing
+    # beyond keep_factor x disk_r_out keep its record without the exact solve (their intensity
+    # is 0 whatever the radii; flagged BH_FLAG_NETWORK_KEPT); 0 solves every ray
+    keep_factor: float = 1.2
 
 
 def ring_tables(cams) -> np.ndarray:
@@ -92,7 +97,7 @@ class SurrogateRenderer:
         self.side = torch.cuda.Stream(device=dev) if split else None  # default priority
         mode = 0 if not cfg.refine else (3 if cfg.solve_band else (2 if split else 1))
         self.params = S.RenderParams(cfg.ring_band, cfg.edge_tol, cfg.conf_max, cfg.ambiguity,
-                                     cfg.audit_every, mode)
+                                     cfg.audit_every, mode, cfg.keep_factor, 0)
         self.floor_I = 0.0
 
     def _stream(self):

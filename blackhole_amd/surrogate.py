@@ -23,7 +23,8 @@ RING_FLOATS = 2 + 3 * RING_BINS
 class RenderParams(C.Structure):
     """BH_SurrogateRenderParams (fallback criteria and audit rate)."""
     _fields_ = [("ring_band", C.c_float), ("edge_tol", C.c_float), ("conf_max", C.c_float),
-                ("ambiguity", C.c_float), ("audit_every", C.c_int32), ("refine", C.c_int32)]
+                ("ambiguity", C.c_float), ("audit_every", C.c_int32), ("refine", C.c_int32),
+                ("keep_factor", C.c_float), ("pad0", C.c_int32)]
 
 
 class SurrogateConfig(C.Structure):

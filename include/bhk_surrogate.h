@@ -128,6 +128,16 @@ typedef struct BH_SurrogateRenderParams {
                              solve starts them from its own cubic Hermite model),
                              the audit sample is drawn from every ray, and only the
                              solver's failures reach the fallback list */
+    float keep_factor;    /* refine 1-3: a ray outside the ring band whose outputs
+                             say escaped or captured, with an unambiguous count and
+                             capture and every predicted crossing at r >= keep_factor
+                             * disk_r_out, keeps the network's record (crossings and
+                             g at them, class, escape direction) without the exact
+                             solve, flagged BH_FLAG_NETWORK_KEPT: its intensity is 0
+                             whatever the crossing radii.  0: every ray is solved.
+                             Ignored when opts->mino_exit = 1 (exact escape
+                             directions asked for) */
+    int32_t pad0;
 } BH_SurrogateRenderParams;
 
 /* Render every pixel of the cameras with the fused surrogate kernel

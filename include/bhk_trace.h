@@ -75,6 +75,10 @@ enum BH_Precision {
 #define BH_FLAG_EXIT_NETWORK  0x20u /* u_exit, phi_exit are the surrogate
                                        network's escape direction (approximate:
                                        neither solved nor integrated)          */
+#define BH_FLAG_NETWORK_KEPT  0x40u /* the surrogate network's record kept
+                                       without the exact solve: class escaped or
+                                       captured with margin, crossing radii
+                                       approximate, all beyond the disk        */
 
 /* ---------------------------------------------------------------------- */
 /* Inputs                                                                  */

@@ -132,23 +132,45 @@ def test_refined_surrogate_render_matches_oracle():
     got = tracer.records_to_numpy(R.out)
     ref = OL.trace(w["cams"], w["opts"], luts=w["luts"], lut_n=w["luts"].shape[1])
     assert np.array_equal(got["hit_class"], ref["hit_class"])
-    _check_disk_records(got, ref, np.ones(len(got), bool))
+    # rays the network places escaped or captured with every predicted crossing
+    # beyond 1.2 r_out keep its record unsolved: right class, intensity 0,
+    # crossing radii to the network's accuracy, all beyond the disk
+    kept = (got["flags"] & abi.FLAG_NETWORK_KEPT) != 0
     refined = (got["flags"] & abi.FLAG_REFINED) != 0
-    assert refined.mean() > 0.9
+    assert not (kept & refined).any() and (kept | refined).mean() > 0.9 and kept.mean() > 0.05
+    _check_disk_records(got, ref, ~kept)
+    assert (got["intensity"][kept] == 0.0).all() and (ref["intensity"][kept] == 0.0).all()
+    # every crossing the oracle records for a kept ray lies beyond the disk
+    # (the safety condition); where both have a k-th crossing, the network's
+    # radius is the network's estimate (median within 1 %)
+    rel = []
+    for k in range(3):
+        sel = kept & (ref["n_cross"] > k)
+        assert (ref["cross_r"][sel, k] > 20.0).all()
+        assert (got["cross_r"][kept & (got["n_cross"] > k), k] >= 24.0).all()
+        both = sel & (got["n_cross"] > k)
+        rel.append(np.abs(got["cross_r"][both, k] / ref["cross_r"][both, k] - 1.0))
+    rel = np.concatenate(rel)
+    print("[refine] kept rays' crossing radii vs the oracle's: median", np.median(rel), "max", rel.max(), flush=True)
+    assert np.median(rel) <= 0.01
+    print("[refine] kept rays with the oracle's crossing count:",
+          float(np.mean(np.minimum(got["n_cross"][kept], 3) == np.minimum(ref["n_cross"][kept], 3))), flush=True)
+    print("[refine] rays kept from the network:", int(kept.sum()), "of", len(got), flush=True)
     fresh = tracer.records_to_numpy(M.out)
-    # escape directions: by default an escaping refined ray keeps the
-    # network's (flagged approximate); the others have none
-    esc = refined & (got["hit_class"] == abi.HIT_ESCAPED)
+    # escape directions: by default an escaping ray keeps the network's
+    # (flagged approximate); the others have none
+    esc = (refined | kept) & (got["hit_class"] == abi.HIT_ESCAPED)
     net_dir = (got["flags"] & abi.FLAG_EXIT_NETWORK) != 0
     assert esc.sum() > 1000 and np.array_equal(net_dir, esc)
     assert np.isfinite(got["u_exit"][esc]).all() and np.isfinite(got["phi_exit"][esc]).all()
     assert np.isnan(got["u_exit"][refined & ~esc]).all() and np.isnan(got["phi_exit"][refined & ~esc]).all()
+    assert np.isfinite(got["u_exit"][kept & esc]).all()
     ang = exit_angle(got, ref, esc)
     print("[refine] network escape direction vs the oracle's: mean", ang.mean(), "p99", np.percentile(ang, 99),
           flush=True)
     assert ang.mean() <= 0.05 and np.percentile(ang, 99) <= 0.3
-    # with exact_exit the refinement computes the exact direction, the same as
-    # the network-free solve's bit for bit (it does not depend on Newton starts)
+    # with exact_exit every ray is solved and gets the exact direction, the same
+    # as the network-free solve's bit for bit (it does not depend on Newton starts)
     X = sur_render.SurrogateRenderer(sur, sc, sur_render.RenderConfig(exact_exit=True))
     X.run(audit=False)
     torch.cuda.synchronize()
@@ -160,7 +182,9 @@ def test_refined_surrogate_render_matches_oracle():
     assert same.sum() > 0.99 * esc_x.sum()
     assert np.array_equal(gx["u_exit"][same], fresh["u_exit"][same])
     assert np.array_equal(gx["phi_exit"][same], fresh["phi_exit"][same])
-    assert np.array_equal(gx["cross_r"], got["cross_r"]) and np.array_equal(gx["intensity"], got["intensity"])
+    assert not ((gx["flags"] & abi.FLAG_NETWORK_KEPT) != 0).any()
+    assert np.array_equal(gx["cross_r"][~kept], got["cross_r"][~kept])
+    assert np.array_equal(gx["intensity"], got["intensity"])
     ang = exit_angle(gx, ref, esc_x)
     print("[refine] exact escape direction (at r_escape) vs the oracle's (at its exit radius): mean", ang.mean(),
           "max", ang.max(), flush=True)
