@@ -284,7 +284,7 @@ def device_frame_flops(rec):
 def bench_c5(args):
     import numpy as np
     import torch
-    from blackhole_amd import flops, sur_render, tracer, workloads
+    from blackhole_amd import abi, flops, sur_render, tracer, workloads
     from blackhole_amd import surrogate_train as ST
 
     w = workloads.c5(args.size)
@@ -326,8 +326,8 @@ def bench_c5(args):
     audit = R.audit_stats()
     acc = frame_accuracy(R.out, plain, w["cams"])
     esc_dir = exit_direction_accuracy(R.out, plain)
-    refined = (R.out[:, 114] & 0x10) != 0
-    n_kept = int(((R.out[:, 114] & 0x40) != 0).sum())  # network records kept unsolved
+    refined = (R.out[:, 114] & abi.FLAG_REFINED) != 0
+    n_kept = int(((R.out[:, 114] & abi.FLAG_NETWORK_KEPT) != 0).sum())  # network records kept unsolved
     iters = R.out.view(torch.int32)[:, 26]
     it_sur = float(iters[refined].double().mean())
     del R

@@ -69,7 +69,9 @@ def ring_tables(cams) -> np.ndarray:
 
 
 class SurrogateRenderer:
-    """Device buffers and launches of the surrogate + fallback pipeline."""
+    """Device buffers and launches of the surrogate + fallback pipeline.  The
+    scene's trace options are copied at construction (with this renderer's
+    escape-direction mode), so later changes to scene.opts do not reach it."""
 
     def __init__(self, sur: S.Surrogate, scene: tracer.DeviceScene, cfg: RenderConfig = RenderConfig(),
                  fallback_frac: float = 1.0):
