@@ -662,14 +662,20 @@ MINO_HD int exit_direction(const Path& p, const PolarTimes& pt, double a, double
   const Jacobi J = jacobi_snd(fma(-twoK, je, ve), pt.mpar);
   const long long ji = static_cast<long long>(je);
   const double sn_e = (ji & 1) ? -J.sn : J.sn;  // sn(v + 2K) = -sn(v)
-  const double c0 = fast_sqrt(fmax(1.0 - pt.s * pt.s, 0.0));
-  const double Pc = pi3_complete(pt.wp, pt.mpar);
-  const double Po = pi3_reduced(pt.wp, pt.mpar, pt.s, c0);
-  const double P0 = pt.toward ? -Po : fma(-2.0, Pc, Po);
-  const double Pe = fma(2.0 * je, Pc, pi3_reduced(pt.wp, pt.mpar, J.sn, J.cn));
+  // xi = 0: the polar term of dphi/dl vanishes (as in the tracer's equation),
+  // while Pi(1; .) would diverge over the pole (w+ = 1)
+  double phi_u = 0.0;
+  if (xi != 0.0) {
+    const double c0 = fast_sqrt(fmax(1.0 - pt.s * pt.s, 0.0));
+    const double Pc = pi3_complete(pt.wp, pt.mpar);
+    const double Po = pi3_reduced(pt.wp, pt.mpar, pt.s, c0);
+    const double P0 = pt.toward ? -Po : fma(-2.0, Pc, Po);
+    const double Pe = fma(2.0 * je, Pc, pi3_reduced(pt.wp, pt.mpar, J.sn, J.cn));
+    phi_u = xi * fast_rcp(pt.k) * (Pe - P0);
+  }
   const double sq_wp = fast_sqrt(pt.wp);
   *u_exit = (u_o > 0.0 ? -sq_wp : sq_wp) * sn_e;
-  *phi_exit = (phi_in + phi_out) + xi * fast_rcp(pt.k) * (Pe - P0);
+  *phi_exit = (phi_in + phi_out) + phi_u;
   return 1;
 }
 

@@ -240,3 +240,29 @@ def test_escape_direction_matches_the_step_integrator(mino):
     print("escape direction vs oracle:", n, "rays; |du| max", du.max(), "|dphi| max", dphi.max(), "mean", dphi.mean())
     assert n > 3000 and fails <= n // 1000
     assert du.max() <= 1e-6 and dphi.max() <= 2e-5 and dphi.mean() <= 1e-6
+
+
+def test_escape_direction_of_meridional_rays(mino):
+    """xi = 0 (the centre column of an odd-width image): the polar term of
+    dphi/dl vanishes, as in the integrator; u_exit and phi_exit still match the
+    oracle's at its exit radius."""
+    w = workloads.c5(25, spins=(0.0, 0.9), inclinations=(30.0, 60.0))
+    o = w["opts"]
+    ref = OL.trace(w["cams"], o, luts=w["luts"], lut_n=w["luts"].shape[1], nthreads=8)
+    r, ints, d = (_D * 3)(), (C.c_int * 6)(), (_D * 2)()
+    n = 0
+    for cam in w["cams"]:
+        i = cam.width // 2
+        al = cam.alpha_min + (i + 0.5) * (cam.alpha_max - cam.alpha_min) / cam.width
+        assert al == 0.0
+        for j in range(cam.height):
+            rec = ref[cam.record_offset + j * cam.width + i]
+            if rec["hit_class"] != 3:
+                continue
+            be = cam.beta_max - (j + 0.5) * (cam.beta_max - cam.beta_min) / cam.height
+            assert mino.mino_solve_pixel_exit(cam.spin, cam.theta_obs, cam.r_obs, al, be,
+                                              o.horizon_factor * tracer.outer_horizon(cam.spin), rec["r_exit"],
+                                              tracer.isco(cam.spin), o.disk_r_out, 1e-10, r, ints, d)
+            assert abs(d[0] - rec["u_exit"]) <= 1e-6 and abs(d[1] - rec["phi_exit"]) <= 1e-6, (cam.spin, be)
+            n += 1
+    assert n > 20
