@@ -115,7 +115,9 @@ __device__ __forceinline__ bool crossing(Lane& L, const DevParams& P, const doub
 
 __device__ __forceinline__ int terminate_check(const Lane& L, const DevParams& P) {
   if (L.y[0] < L.r_stop) return BH_HIT_HORIZON;
-  if (L.y[0] > P.r_escape) return BH_HIT_ESCAPED;
+  // outbound only: a camera beyond r_escape traces its rays in first (inside
+  // r_escape the test equals r > r_escape, no inbound ray can exceed it)
+  if (L.y[0] > P.r_escape && L.y[3] > 0.0) return BH_HIT_ESCAPED;
   if (P.transmit_min > 0.0 && L.rt.Tr < P.transmit_min) return BH_HIT_OPAQUE;
   return 0;
 }

@@ -266,7 +266,7 @@ __device__ __forceinline__ bool crossing_f(LaneF& L, const DevParams& P, const T
 
 __device__ __forceinline__ int terminate_f(const LaneF& L, const DevParams& P) {
   if (L.y[0] < L.r_stop) return BH_HIT_HORIZON;
-  if (static_cast<double>(L.y[0]) > P.r_escape) return BH_HIT_ESCAPED;
+  if (static_cast<double>(L.y[0]) > P.r_escape && L.y[3] > 0.0f) return BH_HIT_ESCAPED;  // outbound
   if (P.transmit_min > 0.0 && static_cast<double>(L.Tr) < P.transmit_min) return BH_HIT_OPAQUE;
   return 0;
 }

@@ -50,7 +50,7 @@ enum BH_HitClass {
     BH_HIT_NONE      = 0, /* ray not traced (e.g. skipped by a mask)            */
     BH_HIT_HORIZON   = 1, /* r < horizon_factor * r_+                           */
     BH_HIT_DISK      = 2, /* crossed the thin disk inside [r_isco, disk_r_out]  */
-    BH_HIT_ESCAPED   = 3, /* r > r_escape                                       */
+    BH_HIT_ESCAPED   = 3, /* r > r_escape, moving outward                       */
     BH_HIT_MAX_STEPS = 4, /* step (RK4) or attempt (DP45) budget exhausted      */
     BH_HIT_OPAQUE    = 5  /* volume transmittance fell below transmit_min       */
 };
@@ -128,7 +128,8 @@ typedef struct BH_TraceOptions {
     double  h_vol;          /* DP45 step cap inside the volume support          */
     double  rtol, atol;     /* DP45 tolerances                                  */
     double  horizon_factor; /* stop when r < horizon_factor * r_+               */
-    double  r_escape;       /* stop when r > r_escape                           */
+    double  r_escape;       /* stop when r > r_escape on the way out (a camera
+                               beyond it traces its rays in first)              */
     double  disk_r_out;     /* thin-disk outer edge [M]; inner edge is r_isco   */
     double  nu_obs;         /* observing frequency at infinity [Hz]             */
     double  disk_T_max;     /* disk effective temperature at the flux peak [K]  */

@@ -6,7 +6,10 @@
 //
 // The per-ray loop restates KerrRaytracer::trace (src/physics/raytracer.h:
 // 324-378) -- capture / escape / step-budget termination -- with the
-// c1 thresholds (r < horizon_factor r_+, r > r_escape), the disk test of
+// c1 thresholds (r < horizon_factor r_+, r > r_escape on the way out: a
+// camera beyond r_escape sees its rays travel in before any can escape; for a
+// camera inside it the two forms agree, since an inbound ray reaches
+// r_escape only after its radial turning point), the disk test of
 // d_trace_geodesic (src/cuda/device_physics.cuh:912-1017) and the
 // front-to-back volumetric transfer of d_trace_geodesic_rte
 // (src/cuda/device_physics.cuh:2037-2173, d_rte_step :1988-2015), all in FP64.
@@ -117,7 +120,7 @@ inline void write_stokes(const RayAcc& acc, double* st) {
 
 inline int terminate_check(const Scene& S, const CamConst& k, const double* y, const RayAcc& acc) {
   if (y[0] < k.r_stop) return BH_HIT_HORIZON;
-  if (y[0] > S.o->r_escape) return BH_HIT_ESCAPED;
+  if (y[0] > S.o->r_escape && y[3] > 0.0) return BH_HIT_ESCAPED;  // outbound
   if (S.o->transmit_min > 0.0 && acc.rt.Tr < S.o->transmit_min) return BH_HIT_OPAQUE;
   return 0;
 }
