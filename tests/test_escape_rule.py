@@ -75,6 +75,8 @@ def mino(tmp_path_factory):
     L = C.CDLL(str(so))
     D = C.c_double
     L.mino_solve_pixel.argtypes = [D] * 9 + [C.POINTER(D), C.c_int, D, C.POINTER(D), C.POINTER(C.c_int)]
+    L.mino_solve_pixel_exit.restype = C.c_int
+    L.mino_solve_pixel_exit.argtypes = [D] * 10 + [C.POINTER(D), C.POINTER(C.c_int), C.POINTER(D)]
     return L
 
 
@@ -83,8 +85,8 @@ def test_mino_solver_matches_the_oracle_for_a_far_camera(mino):
     ref = OL.trace(cams, o, luts=luts, lut_n=luts.shape[1], nthreads=8)
     cam = cams[0]
     D = C.c_double
-    r, ints = (D * 3)(), (C.c_int * 6)()
-    n = 0
+    r, ints, d = (D * 3)(), (C.c_int * 6)(), (D * 2)()
+    n = n_esc = 0
     for j in range(cam.height):
         for i in range(cam.width):
             rec = ref[j * cam.width + i]
@@ -98,5 +100,13 @@ def test_mino_solver_matches_the_oracle_for_a_far_camera(mino):
             assert ints[2] == rec["hit_class"], (i, j)
             for c in range(min(3, ints[0], rec["n_cross"])):
                 assert abs(r[c] - rec["cross_r"][c]) <= 1e-6 * rec["cross_r"][c], (i, j, c)
+            if rec["hit_class"] == abi.HIT_ESCAPED:
+                # the direction at the oracle's own exit radius on the way out
+                assert mino.mino_solve_pixel_exit(cam.spin, cam.theta_obs, cam.r_obs, al, be,
+                                                  o.horizon_factor * tracer.outer_horizon(cam.spin),
+                                                  rec["r_exit"], tracer.isco(cam.spin), o.disk_r_out, 1e-10,
+                                                  r, ints, d)
+                assert abs(d[0] - rec["u_exit"]) <= 1e-6 and abs(d[1] - rec["phi_exit"]) <= 2e-5, (i, j)
+                n_esc += 1
             n += 1
-    assert n >= 250
+    assert n >= 250 and n_esc > 40

@@ -113,6 +113,37 @@ def test_mino_disk_matches_oracle_and_falls_back_exactly():
     assert np.array_equal(after["hit_class"], ref["hit_class"])
 
 
+def test_mino_disk_far_cameras_match_oracle():
+    """Cameras beyond r_escape (r_obs 500 and 900 M against 200 M): the solver's
+    path runs in from the camera and ends at r_escape on the way out, as the
+    integrator's rays do under the outbound escape test; records against the
+    oracle, escape directions against the host build of the solver."""
+    import torch
+    cams = [tracer.camera(a, inc, r_obs, 40, 40, 15.0)
+            for a, inc, r_obs in ((0.3, 30.0, 500.0), (0.9, 70.0, 900.0))]
+    tracer.pack_cameras(cams)
+    w = dict(cams=cams, opts=workloads.c5_options(), luts=tracer.disk_luts(cams, 20.0))
+    print("[mino far] 2 x 40^2 rays, r_obs 500 / 900", flush=True)
+    sc = tracer.upload_scene(w["cams"], w["opts"], w["luts"])
+    R = sur_render.MinoRenderer(sc)
+    R.solve()
+    torch.cuda.synchronize()
+    got = tracer.records_to_numpy(R.out)
+    ref = OL.trace(w["cams"], w["opts"], luts=w["luts"], lut_n=w["luts"].shape[1])
+    flagged = (got["flags"] & abi.FLAG_FALLBACK) != 0
+    assert int(flagged.sum()) == R.fallback_count() <= sc.n_records // 100
+    esc = ~flagged & (got["hit_class"] == abi.HIT_ESCAPED)
+    assert esc.sum() > 500 and (ref["r_exit"][esc] > 200.0).all() and (ref["r_exit"][esc] < 260.0).all()
+    assert (ref["hit_class"] == abi.HIT_DISK).sum() > 1500
+    _check_disk_records(got, ref, ~flagged)
+    host = _host_exit_directions(w, np.nonzero(esc)[0])
+    assert np.isfinite(host).all()
+    du = np.abs(got["u_exit"][esc] - host[:, 0]).max()
+    dphi = np.abs(got["phi_exit"][esc] - host[:, 1]).max()
+    print("[mino far] escape directions, device vs host solver: |du|", du, "|dphi|", dphi, flush=True)
+    assert du <= 1e-9 and dphi <= 1e-9
+
+
 def test_refined_surrogate_render_matches_oracle():
     """Surrogate + refinement + fallback on four of the frames: every record
     at the oracle's class and its intensity; the solver converges in a few
