@@ -61,6 +61,23 @@ def test_fp32_rk4_disk_c1(oracle):
     assert ((gpu["flags"] & abi.FLAG_FP64_ISLAND) != 0).sum() == s["island_rays"]
 
 
+def test_fp32_far_camera_traces_in_before_escaping(oracle):
+    """A camera beyond r_escape (r_obs 600 M against 200 M): the FP32 form's
+    rays travel in and leave through r_escape on the way out, as the
+    oracle's do."""
+    cams = [tracer.camera(0.5, 40.0, 600.0, 96, 96, 15.0)]
+    tracer.pack_cameras(cams)
+    o = workloads.c5_options()
+    o.precision = abi.PRECISION_FP32
+    w = dict(cams=cams, opts=o, luts=tracer.disk_luts(cams, 20.0))
+    gpu, ref, s = _compare(w, oracle, "fp32-far")
+    assert s["class_diff_outside_band"] == 0
+    assert s["mean_rel"] <= TOL_MEAN_I
+    assert (ref["hit_class"] == abi.HIT_DISK).sum() > 4000 and (ref["hit_class"] == abi.HIT_HORIZON).sum() > 300
+    esc = gpu["hit_class"] == abi.HIT_ESCAPED
+    assert esc.sum() > 200 and (gpu["r_exit"][esc] > 200.0).all() and (gpu["r_exit"][esc] < 260.0).all()
+
+
 def test_fp32_dp45_volume_c2(oracle):
     w = workloads.c2(128, 128)
     w["opts"].precision = abi.PRECISION_FP32

@@ -28,6 +28,26 @@ def test_launch_records_match_oracle(spin, pos):
     assert (gpu["hit_class"] == 2).sum() > 0 and (gpu["hit_class"] == 1).sum() > 0
 
 
+def test_camera_beyond_max_dist_renders_the_hole():
+    """max_dist is the escape radius for outbound rays: a camera at 360 M with
+    max_dist = 200 M sees the disk and the shadow (records equal to the
+    oracle's), its escaped rays leave through max_dist on the way out."""
+    import torch
+    from blackhole_amd import tracer
+    p = tracer.launch_params(rs=2.0, spin=0.5, cam_pos=(0.0, -300.0, 200.0), width=48, height=32,
+                             fov_scale=0.03, max_dist=200.0)
+    gpu = tracer.records_to_numpy(tracer.trace_launch_params(p, variant=2))
+    torch.cuda.synchronize()
+    ref = OL.trace_launch_params(p)
+    st = compare(gpu, ref)
+    print("far camera render records", st, np.bincount(gpu["hit_class"], minlength=6), flush=True)
+    assert st["class_mismatch"] == 0 and st["bitwise_exit"] == 1.0
+    assert st["max_rel_intensity"] <= 1e-9
+    assert (gpu["hit_class"] == 2).sum() > 500 and (gpu["hit_class"] == 1).sum() > 50
+    esc = gpu["hit_class"] == 3
+    assert (gpu["r_exit"][esc] > 200.0).all()
+
+
 def test_fp32_variant_classes_and_framebuffer_shading():
     import torch
     from blackhole_amd import abi, tracer
