@@ -263,7 +263,12 @@ int bh_trace_image_plane(const BH_Camera* cams, int n_cameras,
  * cameras per call; larger launches return -1 with the limit in
  * bh_last_error()).  All calls on one GPU share that array: calls on one
  * stream are ordered, calls that overlap on one GPU (other streams or host
- * threads) race.  bh_last_kernel_ms() does not time these launches. */
+ * threads) race.  bh_last_kernel_ms() does not time these launches.
+ * The launch block is uploaded from a per-host-thread staging buffer by an
+ * asynchronous copy; a copy captured into a HIP graph reads that buffer when
+ * the graph runs, so a captured call replays its own launch only while no
+ * other call of this function on the same host thread comes in between
+ * (BH_Frame is the form built for graph replay). */
 int bh_trace_image_plane_noalloc(const BH_Camera* cams, int n_cameras,
                                  const BH_TraceOptions* opts,
                                  const double* disk_luts, int disk_lut_n,
