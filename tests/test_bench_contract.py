@@ -48,3 +48,22 @@ def test_measured_peaks_attach_to_every_roofline():
     assert res["legs"][0]["roofline"]["frac_measured"] == 0.5
     assert "peak_measured" not in res["other"]
     assert res["measured_peaks"] is pk
+
+
+def test_product_path_fails_loudly_without_the_library():
+    """No CPU fallback: with libbhk.so absent, the first call into the
+    product raises NativeLibraryMissing (run in a child process so the
+    loaded library of this one is untouched)."""
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from blackhole_amd import _lib, tracer\n"
+        "_lib.LIB_PATH = '/nonexistent/libbhk.so'\n"
+        "try:\n"
+        "    tracer.outer_horizon(0.5)\n"
+        "except _lib.NativeLibraryMissing as e:\n"
+        "    print('raised:', e)\n"
+        "    sys.exit(0)\n"
+        "sys.exit(3)\n" % ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    assert "is not built" in r.stdout
